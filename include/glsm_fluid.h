@@ -1,0 +1,151 @@
+/* glsm_fluid.h -- C ABI of the MI355X gLSM_fluid_3D time step.
+ *
+ * The reference exposes this path as three C++ classes (src/gel.h,
+ * src/fluid.h, src/coupling.h).  This header is the flat C binding of those
+ * classes: every entry point below replaces one constructor or method (cited
+ * next to it) with plain pointers, sizes and an int status.  The same classes
+ * are also available as C++ (gel.h, fluid.h, coupling.h in this directory),
+ * which is what the sim*.cpp drivers compile against.
+ *
+ * Conventions
+ *   - return 0 on success, a negative GLSM_E* code on failure;
+ *     glsm_last_error() returns the message of the calling thread's last
+ *     failure ("" if none).
+ *   - all work is enqueued on one library stream in call order (the
+ *     reference's per-object streams are unordered; program order is what its
+ *     drivers assume).  Reads synchronise that stream before returning.
+ *   - array reads/writes copy whole fields between device and host memory;
+ *     `bytes` must equal glsm_*_field_bytes() for that field.
+ */
+#ifndef GLSM_FLUID_H_
+#define GLSM_FLUID_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum glsm_status {
+  GLSM_OK = 0,
+  GLSM_EINVAL = -1,   /* bad argument (null handle, size mismatch, range) */
+  GLSM_EDEVICE = -2,  /* HIP runtime error */
+  GLSM_ESTATE = -3,   /* call not valid in the object's current state */
+  GLSM_EIO = -4,      /* file could not be read or written */
+  GLSM_ENOMEM = -5
+};
+
+/* ---- gel fields (layouts are the reference's: element arrays (L+1)^3,
+ *      node arrays (L+2)^3, x fastest; double3 stored as 3 doubles) ---- */
+enum glsm_gel_field {
+  GLSM_GEL_UM = 0, GLSM_GEL_VM, GLSM_GEL_WM, GLSM_GEL_UM_NORM, GLSM_GEL_VM_NORM,
+  GLSM_GEL_WMP, GLSM_GEL_T0, GLSM_GEL_T1, GLSM_GEL_T2, GLSM_GEL_VOL, GLSM_GEL_PM,
+  GLSM_GEL_RM,          /* element double3 */
+  GLSM_GEL_RM_LOC,      /* element double3 x 8 */
+  GLSM_GEL_NMSM,        /* element double3 x 6 */
+  GLSM_GEL_RN, GLSM_GEL_VELN, GLSM_GEL_VELS, GLSM_GEL_FN, GLSM_GEL_FN_ROBIN,
+  GLSM_GEL_FDRAG_ROBIN, /* node double3 */
+  GLSM_GEL_UN_NORM, GLSM_GEL_UN_ROBIN, GLSM_GEL_VN_NORM, /* node double */
+  GLSM_GEL_MAP_NODE, GLSM_GEL_MAP_ELEMENT, GLSM_GEL_BINDEX, /* int32 */
+  GLSM_GEL_CENTER_VM, GLSM_GEL_CENTER_WM,               /* double[1000] */
+  GLSM_GEL_CENTER_RM, GLSM_GEL_CENTER_FN, GLSM_GEL_CENTER_VELN, /* double3[1000] */
+  GLSM_GEL_FILAMENT,    /* double3[1000 * maxFilamentlen] */
+  GLSM_GEL_NFIELDS
+};
+
+/* ---- coupler fields (marker arrays of length M, gel-major) ---- */
+enum glsm_coupler_field {
+  GLSM_CP_LAG = 0, GLSM_CP_UL, GLSM_CP_VL, GLSM_CP_FDRAG, GLSM_CP_FREP, /* float3 */
+  GLSM_CP_CL, GLSM_CP_DL, GLSM_CP_SL,                                   /* float */
+  GLSM_CP_OWNER, GLSM_CP_BINDEX,                                        /* int32 */
+  GLSM_CP_NFIELDS
+};
+
+/* ---- fluid fields (lattice arrays of length N = Nx*Ny*Nz, x fastest) ---- */
+enum glsm_fluid_field {
+  GLSM_FL_U = 0,   /* float3: velocity of the last sub-step (reference d_u) */
+  GLSM_FL_C,       /* float : concentration (reference d_c1) */
+  GLSM_FL_F,       /* float[19][N]: pre-collision populations f_k(x,q) of the
+                      next sub-step (reference d_f after the swap) */
+  GLSM_FL_FPOST,   /* float[19][N]: post-collision populations (the stored
+                      state of the pull scheme) */
+  GLSM_FL_NFIELDS
+};
+
+typedef struct glsm_gel glsm_gel;
+typedef struct glsm_coupler glsm_coupler;
+typedef struct glsm_fluid glsm_fluid;
+
+/* library */
+int glsm_init(int device);               /* cudaSetDevice (sim.cpp:19) + stream */
+int glsm_synchronize(void);              /* cudaDeviceSynchronize (sim.cpp:85) */
+const char* glsm_last_error(void);
+const char* glsm_version(void);
+
+/* ---- Gel: src/gel.h:33-165 ---- */
+/* Gel::Gel(int3 gelSize, double3 gelPosition, int gelType, int gel_id, int time)
+   gel.cu:555-663 */
+int glsm_gel_create(int size_x, int size_y, int size_z, double pos_x, double pos_y, double pos_z,
+                    int gel_type, int gel_id, int time, glsm_gel** out);
+int glsm_gel_destroy(glsm_gel* g);                          /* Gel::~Gel gel.cu:665 */
+int glsm_gel_step_elasticity(glsm_gel* g, int iter);        /* gel.cu:682-698 */
+int glsm_gel_step_chemistry(glsm_gel* g, int iter);         /* gel.cu:700-704 */
+int glsm_gel_record_center(glsm_gel* g, int iter);          /* gel.cu:706-713 */
+int glsm_gel_write_files(glsm_gel* g, int iter);            /* gel.cu:543-553 */
+int glsm_gel_reset_to_quiescent(glsm_gel* g);               /* gel.cu:731-747 */
+int glsm_gel_fire_excitation_pulse(glsm_gel* g);            /* gel.cu:751-783 */
+int glsm_gel_set_anchor_z(glsm_gel* g, double anchor_z);    /* gel.cu:788-795 */
+int glsm_gel_build_tube_mask(glsm_gel* g, int wall_thickness, int circular); /* gel.cu:811-867 */
+int glsm_gel_get_index(glsm_gel* g, int xi, int yi, int zi, int size);       /* gel.cu:116-119 */
+/* node grid (L), array sizes and marker count (gel.cu:605-610) */
+int glsm_gel_dims(glsm_gel* g, int node_grid[3], int* num_nodes, int* num_elements,
+                  int* boundary_count);
+/* Extension (no reference counterpart): interior elements in z,y,x order get
+   u += amp*U(-1,1), then v += amp*U(-1,1), drawn from splitmix64(seed). */
+int glsm_gel_perturb_chemistry(glsm_gel* g, uint64_t seed, double amplitude);
+size_t glsm_gel_field_bytes(glsm_gel* g, int field);
+int glsm_gel_read(glsm_gel* g, int field, void* host, size_t bytes);
+int glsm_gel_write(glsm_gel* g, int field, const void* host, size_t bytes);
+
+/* ---- Coupler: src/coupling.h:14-58 ---- */
+/* Coupler::Coupler(std::vector<Gel*>&) coupling.cu:121-151 */
+int glsm_coupler_create(glsm_gel* const* gels, int num_gels, glsm_coupler** out);
+int glsm_coupler_destroy(glsm_coupler* c);                  /* coupling.cu:232-235 */
+int glsm_coupler_pack_from_gels(glsm_coupler* c);           /* coupling.cu:153-169 */
+int glsm_coupler_scatter_to_gels(glsm_coupler* c);          /* coupling.cu:171-189 */
+int glsm_coupler_apply_gel_repulsion(glsm_coupler* c);      /* coupling.cu:191-194 */
+int glsm_coupler_marker_count(glsm_coupler* c);             /* sumGelBoundaryCount */
+size_t glsm_coupler_field_bytes(glsm_coupler* c, int field);
+int glsm_coupler_read(glsm_coupler* c, int field, void* host, size_t bytes);
+
+/* ---- Fluid: src/fluid.h:24-84 ---- */
+/* Fluid::Fluid(int3 fluidSize, int time, Coupler*) fluid.cu:152-232 */
+int glsm_fluid_create(int size_x, int size_y, int size_z, int time, glsm_coupler* c,
+                      glsm_fluid** out);
+int glsm_fluid_destroy(glsm_fluid* f);                      /* fluid.cu:234-237 */
+int glsm_fluid_step_velocity(glsm_fluid* f, int iter);      /* fluid.cu:263-284 */
+int glsm_fluid_step_concentration(glsm_fluid* f, int iter); /* fluid.cu:251-261 */
+int glsm_fluid_write_files(glsm_fluid* f, int iter);        /* fluid.cu:140-150 */
+/* FluidParams::perX/perY/perZ (fluid.h:21); honoured by stream, IBM and scalar */
+int glsm_fluid_set_periodic(glsm_fluid* f, int per_x, int per_y, int per_z);
+/* FluidParams::beta and the IBM ramp length, normally from $IBM_BETA /
+   $IBM_RAMP_STEPS (fluid.cu:198-201, 265-268) */
+int glsm_fluid_set_ibm(glsm_fluid* f, float beta, float ramp_steps);
+/* lattice (Nx,Ny,Nz) and Nsub (fluid.cu:167-197) */
+int glsm_fluid_dims(glsm_fluid* f, int grid[3], int* nsub);
+size_t glsm_fluid_field_bytes(glsm_fluid* f, int field);
+int glsm_fluid_read(glsm_fluid* f, int field, void* host, size_t bytes);
+int glsm_fluid_write(glsm_fluid* f, int field, const void* host, size_t bytes);
+
+/* ---- one time step in sim.cpp's order (sim.cpp:66-81): for each gel
+   stepElasticity, stepChemistry, recordCenter; packFromGels; stepVelocity;
+   stepConcentration; applyGelRepulsion; scatterToGels.  The gels must be the
+   ones the coupler was built from, in the same order. ---- */
+int glsm_system_step(glsm_gel* const* gels, int num_gels, glsm_coupler* c, glsm_fluid* f,
+                     int iter);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GLSM_FLUID_H_ */
