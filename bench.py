@@ -1,0 +1,198 @@
+#!/usr/bin/env python3
+"""Benchmark of the gLSM_fluid_3D time step on one MI355X.
+
+Workload (one "step" = one gel time step: every gel's elasticity/chemistry,
+marker pack, 25 LBM sub-steps with immersed-boundary coupling, the
+concentration step, repulsion and scatter -- sim.cpp's loop body):
+  101^3 D3Q19 lattice (physical box 50^3, h = 0.5, walls) with the
+  concentration scalar, two 6x6x6-element gels at x = 1/3 and 2/3 of the box,
+  centred in y and z (sim.cpp's fractional placement), reference constants,
+  chemistry at steady state + splitmix64(seed 1) perturbation of 0.01,
+  FP32 arithmetic with FP32 deviation storage.
+
+Metric: MLUPS = lattice cells x 25 sub-steps x steps / wall seconds / 1e6,
+inputs resident in HBM, no file output inside the timed region.
+
+Printed JSON line (contract): metric, value, unit, n_gpus, steps, warmup,
+ms_per_step, roofline (stream/collide kernel, live HIP-event timing),
+cpu_baseline (the FP32 OpenMP restatement on a bounded sample of the same
+workload), and extra lines for the 201^3 four-gel lattice in FP32 and FP16
+deviation storage.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+NSUB = 25
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def config1(mod, storage=None, prec=None, seed=1):
+    """101^3 lattice, two 6^3 gels placed as in sim.cpp (x = 1/3, 2/3)."""
+    box = (50, 50, 50)
+    gels = [mod.Gel((6, 6, 6), (box[0] / 3.0, box[1] / 2.0, box[2] / 2.0), gel_id=1),
+            mod.Gel((6, 6, 6), (2.0 * box[0] / 3.0, box[1] / 2.0, box[2] / 2.0), gel_id=2)]
+    for i, g in enumerate(gels):
+        g.perturb_chemistry(seed + i, 0.01)
+    c = mod.Coupler(gels) if prec is None else mod.Coupler(gels, prec)
+    f = mod.Fluid(box, c)
+    if storage:
+        f.set_storage(storage)
+    return gels, c, f
+
+
+def four_gel_centres(seed=2, box=100.0, n=8, count=4):
+    """Seeded uniform centres of n^3-element gels, rejected when their boxes
+    come closer than one element (the 201^3 precision workload)."""
+    rng = np.random.default_rng(seed)
+    half = n / 2.0
+    out = []
+    while len(out) < count:
+        p = rng.uniform(half + 3, box - half - 3, 3)
+        if all(np.any(np.abs(p - q) >= n + 1) for q in out):
+            out.append(p)
+    return [tuple(float(v) for v in p) for p in out]
+
+
+def config2b(mod, storage):
+    gels = [mod.Gel((8, 8, 8), p, gel_id=i + 1) for i, p in enumerate(four_gel_centres())]
+    for i, g in enumerate(gels):
+        g.perturb_chemistry(2 + i, 0.01)
+    c = mod.Coupler(gels)
+    f = mod.Fluid((100, 100, 100), c)
+    f.set_storage(storage)
+    return gels, c, f
+
+
+def run_gpu(G, gels, c, f, steps, warmup, every=10):
+    for it in range(warmup):
+        G.system_step(gels, c, f, it)
+    f.enable_lbm_timing(every)
+    G.synchronize()
+    t0 = time.perf_counter()
+    for it in range(warmup, warmup + steps):
+        G.system_step(gels, c, f, it)
+    G.synchronize()
+    dt = time.perf_counter() - t0
+    lbm_ms, samples = f.lbm_timing()
+    return dt, lbm_ms, samples
+
+
+def roofline(N, bytes_per_cell, lbm_ms, traffic=None):
+    alg_bytes = N * bytes_per_cell
+    achieved = alg_bytes / (lbm_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "kernel": "k_lbm_step (fused pull stream + BGK/Guo collide)",
+            "bytes_per_launch": alg_bytes, "mean_launch_ms": round(lbm_ms, 5)}
+
+
+def load_traffic(name):
+    """HBM bytes per launch from the committed rocprofv3 counter summary."""
+    p = os.path.join(ROOT, "profiles", "lbm_traffic.json")
+    if not os.path.exists(p):
+        return None
+    try:
+        return json.load(open(p)).get(name)
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(target_s):
+    import oracle as O
+
+    threads = min(16, os.cpu_count() or 1)
+    O.set_threads(threads)
+    gels, c, f = config1(O, prec=32)
+    N = f.N
+    t0 = time.perf_counter()
+    O.system_step(gels, c, f, 0)
+    first = time.perf_counter() - t0
+    steps = max(1, int(target_s / max(first, 1e-3)) - 1)
+    t0 = time.perf_counter()
+    for it in range(1, 1 + steps):
+        O.system_step(gels, c, f, it)
+    dt = time.perf_counter() - t0
+    return {"value": round(N * NSUB * steps / dt / 1e6, 2), "unit": "MLUPS", "cores": threads, "kind": "port",
+            "sample": f"FP32 OpenMP restatement (oracle/), same 101^3 two-gel workload, steps 1..{steps} "
+                      f"after one untimed step ({N * NSUB * steps / 1e6:.0f} M lattice updates, {dt:.1f} s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--storage", default="f32", choices=["f32", "f32full", "f16", "bf16"])
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--extra-steps", type=int, default=400)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for {args.gpus}); the time step runs on a single "
+              "device and multi-GPU runs are not part of this build.", file=sys.stderr)
+        sys.exit(2)
+
+    import glsm_fluid_3d_amd as G
+
+    G.init(0)
+    gels, c, f = config1(G, storage=args.storage)
+    N = f.N
+    log(f"workload 101^3: N={N} nsub={f.nsub} markers={c.M} storage={args.storage}")
+    dt, lbm_ms, samples = run_gpu(G, gels, c, f, args.steps, args.warmup)
+    mlups = N * f.nsub * args.steps / dt / 1e6
+    bpc = 2 * 19 * G.STORAGE_BYTES[args.storage]
+    line = {
+        "metric": "MLUPS (LBM lattice-node updates/s, full gel-fluid time step)",
+        "value": round(mlups, 1), "unit": "MLUPS", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(dt / args.steps * 1e3, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "101^3 D3Q19 + concentration, two 6x6x6-element gels at x=1/3,2/3 (sim.cpp "
+                               "placement), walls, seeded chemistry perturbation, 25 LBM sub-steps per gel step",
+                   "lattice": [101, 101, 101], "gels": 2, "markers": c.M, "nsub": f.nsub,
+                   "storage": args.storage + (" (f - w_q deviations)" if args.storage != "f32full" else ""),
+                   "parallelism": "single GPU"},
+        "roofline": roofline(N, bpc, lbm_ms, load_traffic(f"config1_{args.storage}")),
+    }
+    line["roofline"]["samples"] = samples
+    log(f"101^3: {mlups:.0f} MLUPS, {dt / args.steps * 1e3:.3f} ms/step, lbm kernel {lbm_ms * 1e3:.1f} us")
+    for x in gels + [c, f]:
+        x.close()
+
+    if not args.no_extra:
+        extra = {}
+        for st in ("f32", "f16"):
+            gels, c, f = config2b(G, st)
+            dt2, ms2, n2 = run_gpu(G, gels, c, f, args.extra_steps, 20)
+            v2 = f.N * f.nsub * args.extra_steps / dt2 / 1e6
+            extra[f"201cube_4gels_{st}"] = {
+                "value": round(v2, 1), "unit": "MLUPS", "steps": args.extra_steps,
+                "ms_per_step": round(dt2 / args.extra_steps * 1e3, 4),
+                "roofline": roofline(f.N, 2 * 19 * G.STORAGE_BYTES[st], ms2, load_traffic(f"config2b_{st}"))}
+            log(f"201^3 {st}: {v2:.0f} MLUPS, lbm kernel {ms2 * 1e3:.1f} us")
+            for x in gels + [c, f]:
+                x.close()
+        line["extra"] = extra
+
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        line["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

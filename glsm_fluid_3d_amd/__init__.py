@@ -1,0 +1,325 @@
+"""MI355X-native gLSM_fluid_3D time step -- Python binding of the C ABI.
+
+The compute lives in ``libglsm_fluid.so`` (hand-written HIP for gfx950, built
+in-tree by ``glsm_fluid_3d_amd.build``).  This module only loads it with
+ctypes and mirrors the reference's ``Gel`` / ``Coupler`` / ``Fluid`` classes
+(src/gel.h:33-165, src/coupling.h:14-58, src/fluid.h:24-84): same
+constructor arguments, step methods and phase controls, plus whole-field
+reads and writes.  There is no fallback: if the library is missing or no GPU
+is present, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libglsm_fluid.so")
+
+GEL_FIELDS = [
+    "um", "vm", "wm", "um_norm", "vm_norm", "wmp", "T0", "T1", "T2", "vol", "pm",
+    "rm", "rm_loc", "nmSm", "rn", "Veln", "Vels", "Fn", "Fn_robin", "Fdrag_robin",
+    "un_norm", "un_robin", "vn_norm", "map_node", "map_element", "bIndex",
+    "center_vm", "center_wm", "center_rm", "center_Fn", "center_Veln", "filament",
+]
+GEL_FIELD = {n: i for i, n in enumerate(GEL_FIELDS)}
+CP_FIELDS = ["lag", "Ul", "Vl", "Fdrag", "Frep", "Cl", "Dl", "Sl", "owner", "bIndex"]
+CP_FIELD = {n: i for i, n in enumerate(CP_FIELDS)}
+FL_FIELDS = ["u", "c", "f", "fpost"]
+FL_FIELD = {n: i for i, n in enumerate(FL_FIELDS)}
+STORAGE = {"f32full": 0, "f32": 1, "f16": 2, "bf16": 3}
+STORAGE_BYTES = {"f32full": 4, "f32": 4, "f16": 2, "bf16": 2}
+
+# every entry point of include/glsm_fluid.h: name -> (restype, argtypes)
+_vp, _i, _d, _sz, _f = C.c_void_p, C.c_int, C.c_double, C.c_size_t, C.c_float
+_pi = C.POINTER(C.c_int)
+SIGNATURES = {
+    "glsm_init": (_i, [_i]),
+    "glsm_synchronize": (_i, []),
+    "glsm_last_error": (C.c_char_p, []),
+    "glsm_version": (C.c_char_p, []),
+    "glsm_gel_create": (_i, [_i, _i, _i, _d, _d, _d, _i, _i, _i, C.POINTER(_vp)]),
+    "glsm_gel_destroy": (_i, [_vp]),
+    "glsm_gel_step_elasticity": (_i, [_vp, _i]),
+    "glsm_gel_step_chemistry": (_i, [_vp, _i]),
+    "glsm_gel_record_center": (_i, [_vp, _i]),
+    "glsm_gel_write_files": (_i, [_vp, _i]),
+    "glsm_gel_reset_to_quiescent": (_i, [_vp]),
+    "glsm_gel_fire_excitation_pulse": (_i, [_vp]),
+    "glsm_gel_set_anchor_z": (_i, [_vp, _d]),
+    "glsm_gel_build_tube_mask": (_i, [_vp, _i, _i]),
+    "glsm_gel_get_index": (_i, [_vp, _i, _i, _i, _i]),
+    "glsm_gel_dims": (_i, [_vp, _pi, _pi, _pi, _pi]),
+    "glsm_gel_perturb_chemistry": (_i, [_vp, C.c_uint64, _d]),
+    "glsm_gel_field_bytes": (_sz, [_vp, _i]),
+    "glsm_gel_read": (_i, [_vp, _i, _vp, _sz]),
+    "glsm_gel_write": (_i, [_vp, _i, _vp, _sz]),
+    "glsm_coupler_create": (_i, [C.POINTER(_vp), _i, C.POINTER(_vp)]),
+    "glsm_coupler_destroy": (_i, [_vp]),
+    "glsm_coupler_pack_from_gels": (_i, [_vp]),
+    "glsm_coupler_scatter_to_gels": (_i, [_vp]),
+    "glsm_coupler_apply_gel_repulsion": (_i, [_vp]),
+    "glsm_coupler_marker_count": (_i, [_vp]),
+    "glsm_coupler_field_bytes": (_sz, [_vp, _i]),
+    "glsm_coupler_read": (_i, [_vp, _i, _vp, _sz]),
+    "glsm_fluid_create": (_i, [_i, _i, _i, _i, _vp, C.POINTER(_vp)]),
+    "glsm_fluid_destroy": (_i, [_vp]),
+    "glsm_fluid_step_velocity": (_i, [_vp, _i]),
+    "glsm_fluid_step_concentration": (_i, [_vp, _i]),
+    "glsm_fluid_sub_step": (_i, [_vp, _f]),
+    "glsm_fluid_write_files": (_i, [_vp, _i]),
+    "glsm_fluid_set_periodic": (_i, [_vp, _i, _i, _i]),
+    "glsm_fluid_set_ibm": (_i, [_vp, _f, _f]),
+    "glsm_fluid_set_storage": (_i, [_vp, _i, _i]),
+    "glsm_fluid_enable_lbm_timing": (_i, [_vp, _i, _i]),
+    "glsm_fluid_lbm_timing": (_i, [_vp, C.POINTER(_d), _pi]),
+    "glsm_fluid_dims": (_i, [_vp, _pi, _pi]),
+    "glsm_fluid_field_bytes": (_sz, [_vp, _i]),
+    "glsm_fluid_read": (_i, [_vp, _i, _vp, _sz]),
+    "glsm_fluid_write": (_i, [_vp, _i, _vp, _sz]),
+    "glsm_system_step": (_i, [C.POINTER(_vp), _i, _vp, _vp, _i]),
+}
+
+_lib = None
+
+
+class GlsmError(RuntimeError):
+    pass
+
+
+def compile_library() -> str:
+    """Compile libglsm_fluid.so in-tree (see glsm_fluid_3d_amd/build.py)."""
+    import importlib
+
+    return importlib.import_module(__name__ + ".build").build()
+
+
+def load(path: str = LIB_PATH):
+    """Load libglsm_fluid.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(path):
+            raise GlsmError(f"{path} is missing: run `python -m glsm_fluid_3d_amd.build` first")
+        L = C.CDLL(path)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc: int, what: str):
+    if rc < 0:
+        msg = load().glsm_last_error().decode(errors="replace")
+        raise GlsmError(f"{what} failed ({rc}): {msg}")
+    return rc
+
+
+def init(device: int = 0):
+    _check(load().glsm_init(int(device)), "glsm_init")
+
+
+def synchronize():
+    _check(load().glsm_synchronize(), "glsm_synchronize")
+
+
+def gel_field_shape(name, nN, nE, M, maxfil=300):
+    _ELEM_D = {"um", "vm", "wm", "um_norm", "vm_norm", "wmp", "T0", "T1", "T2", "vol", "pm"}
+    _NODE_D3 = {"rn", "Veln", "Vels", "Fn", "Fn_robin", "Fdrag_robin"}
+    if name in _ELEM_D:
+        return np.float64, (nE,)
+    if name == "rm":
+        return np.float64, (nE, 3)
+    if name == "rm_loc":
+        return np.float64, (nE * 8, 3)
+    if name == "nmSm":
+        return np.float64, (nE * 6, 3)
+    if name in _NODE_D3:
+        return np.float64, (nN, 3)
+    if name in ("un_norm", "un_robin", "vn_norm"):
+        return np.float64, (nN,)
+    if name == "map_node":
+        return np.int32, (nN,)
+    if name == "map_element":
+        return np.int32, (nE,)
+    if name == "bIndex":
+        return np.int32, (M,)
+    if name in ("center_vm", "center_wm"):
+        return np.float64, (1000,)
+    if name in ("center_rm", "center_Fn", "center_Veln"):
+        return np.float64, (1000, 3)
+    if name == "filament":
+        return np.float64, (1000 * maxfil, 3)
+    raise KeyError(name)
+
+
+class Gel:
+    """Gel(gelSize, gelPosition, gelType, gel_id, time) -- src/gel.h:35."""
+
+    def __init__(self, size, pos, gel_type=1, gel_id=1, time=0):
+        h = C.c_void_p()
+        _check(load().glsm_gel_create(int(size[0]), int(size[1]), int(size[2]), float(pos[0]),
+                                      float(pos[1]), float(pos[2]), int(gel_type), int(gel_id),
+                                      int(time), C.byref(h)), "glsm_gel_create")
+        self.h = h.value
+        g3 = (C.c_int * 3)()
+        nN, nE, M = C.c_int(), C.c_int(), C.c_int()
+        _check(load().glsm_gel_dims(self.h, g3, C.byref(nN), C.byref(nE), C.byref(M)), "glsm_gel_dims")
+        self.node_grid = tuple(g3)
+        self.num_nodes, self.num_elements, self.boundary_count = nN.value, nE.value, M.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().glsm_gel_destroy(self.h)
+            self.h = None
+
+    def step_elasticity(self, it):
+        _check(load().glsm_gel_step_elasticity(self.h, int(it)), "step_elasticity")
+
+    def step_chemistry(self, it):
+        _check(load().glsm_gel_step_chemistry(self.h, int(it)), "step_chemistry")
+
+    def record_center(self, it):
+        _check(load().glsm_gel_record_center(self.h, int(it)), "record_center")
+
+    def write_files(self, it):
+        _check(load().glsm_gel_write_files(self.h, int(it)), "write_files")
+
+    def reset_to_quiescent(self):
+        _check(load().glsm_gel_reset_to_quiescent(self.h), "reset_to_quiescent")
+
+    def fire_excitation_pulse(self):
+        _check(load().glsm_gel_fire_excitation_pulse(self.h), "fire_excitation_pulse")
+
+    def set_anchor_z(self, z):
+        _check(load().glsm_gel_set_anchor_z(self.h, float(z)), "set_anchor_z")
+
+    def build_tube_mask(self, wall, circular=False):
+        _check(load().glsm_gel_build_tube_mask(self.h, int(wall), int(bool(circular))), "build_tube_mask")
+
+    def perturb_chemistry(self, seed, amplitude):
+        _check(load().glsm_gel_perturb_chemistry(self.h, int(seed), float(amplitude)), "perturb_chemistry")
+
+    def get_index(self, xi, yi, zi, size):
+        return _check(load().glsm_gel_get_index(self.h, xi, yi, zi, size), "get_index")
+
+    def read(self, name):
+        dt, shape = gel_field_shape(name, self.num_nodes, self.num_elements, self.boundary_count)
+        a = np.empty(shape, dtype=dt)
+        _check(load().glsm_gel_read(self.h, GEL_FIELD[name], a.ctypes.data, a.nbytes), f"gel read {name}")
+        return a
+
+    def write(self, name, arr):
+        dt, shape = gel_field_shape(name, self.num_nodes, self.num_elements, self.boundary_count)
+        a = np.ascontiguousarray(np.asarray(arr, dtype=dt).reshape(shape))
+        _check(load().glsm_gel_write(self.h, GEL_FIELD[name], a.ctypes.data, a.nbytes), f"gel write {name}")
+
+
+class Coupler:
+    """Coupler(std::vector<Gel*>&) -- src/coupling.h:16."""
+
+    def __init__(self, gels):
+        self.gels = list(gels)
+        arr = (C.c_void_p * len(self.gels))(*[g.h for g in self.gels])
+        h = C.c_void_p()
+        _check(load().glsm_coupler_create(arr, len(self.gels), C.byref(h)), "glsm_coupler_create")
+        self.h = h.value
+        self.M = _check(load().glsm_coupler_marker_count(self.h), "marker_count")
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().glsm_coupler_destroy(self.h)
+            self.h = None
+
+    def pack_from_gels(self):
+        _check(load().glsm_coupler_pack_from_gels(self.h), "pack_from_gels")
+
+    def scatter_to_gels(self):
+        _check(load().glsm_coupler_scatter_to_gels(self.h), "scatter_to_gels")
+
+    def apply_gel_repulsion(self):
+        _check(load().glsm_coupler_apply_gel_repulsion(self.h), "apply_gel_repulsion")
+
+    def read(self, name):
+        if name in ("lag", "Ul", "Vl", "Fdrag", "Frep"):
+            a = np.empty((self.M, 3), dtype=np.float32)
+        elif name in ("owner", "bIndex"):
+            a = np.empty((self.M,), dtype=np.int32)
+        else:
+            a = np.empty((self.M,), dtype=np.float32)
+        _check(load().glsm_coupler_read(self.h, CP_FIELD[name], a.ctypes.data, a.nbytes), f"coupler read {name}")
+        return a
+
+
+class Fluid:
+    """Fluid(int3 fluidSize, int time, Coupler*) -- src/fluid.h:26."""
+
+    def __init__(self, size, coupler: Coupler, time=0):
+        self.coupler = coupler
+        h = C.c_void_p()
+        _check(load().glsm_fluid_create(int(size[0]), int(size[1]), int(size[2]), int(time), coupler.h,
+                                        C.byref(h)), "glsm_fluid_create")
+        self.h = h.value
+        g3 = (C.c_int * 3)()
+        ns = C.c_int()
+        _check(load().glsm_fluid_dims(self.h, g3, C.byref(ns)), "glsm_fluid_dims")
+        self.grid = tuple(g3)
+        self.nsub = ns.value
+        self.N = g3[0] * g3[1] * g3[2]
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().glsm_fluid_destroy(self.h)
+            self.h = None
+
+    def step_velocity(self, it):
+        _check(load().glsm_fluid_step_velocity(self.h, int(it)), "step_velocity")
+
+    def step_concentration(self, it):
+        _check(load().glsm_fluid_step_concentration(self.h, int(it)), "step_concentration")
+
+    def sub_step(self, beta_eff):
+        _check(load().glsm_fluid_sub_step(self.h, float(beta_eff)), "sub_step")
+
+    def write_files(self, it):
+        _check(load().glsm_fluid_write_files(self.h, int(it)), "write_files")
+
+    def set_periodic(self, px, py, pz):
+        _check(load().glsm_fluid_set_periodic(self.h, int(px), int(py), int(pz)), "set_periodic")
+
+    def set_ibm(self, beta, ramp_steps):
+        _check(load().glsm_fluid_set_ibm(self.h, float(beta), float(ramp_steps)), "set_ibm")
+
+    def set_storage(self, storage, scale_exp=20):
+        """storage: 'f32' (deviations, default), 'f32full', 'f16', 'bf16' or the enum value."""
+        code = STORAGE[storage] if isinstance(storage, str) else int(storage)
+        _check(load().glsm_fluid_set_storage(self.h, code, int(scale_exp)), "set_storage")
+
+    def enable_lbm_timing(self, every=1, max_samples=4096):
+        _check(load().glsm_fluid_enable_lbm_timing(self.h, int(every), int(max_samples)), "enable_lbm_timing")
+
+    def lbm_timing(self):
+        """(mean stream/collide launch duration in ms, number of samples)."""
+        ms, n = C.c_double(), C.c_int()
+        _check(load().glsm_fluid_lbm_timing(self.h, C.byref(ms), C.byref(n)), "lbm_timing")
+        return ms.value, n.value
+
+    def read(self, name):
+        shape = {"u": (self.N, 3), "c": (self.N,), "f": (19, self.N), "fpost": (19, self.N)}[name]
+        a = np.empty(shape, dtype=np.float32)
+        _check(load().glsm_fluid_read(self.h, FL_FIELD[name], a.ctypes.data, a.nbytes), f"fluid read {name}")
+        return a
+
+    def write(self, name, arr):
+        shape = {"u": (self.N, 3), "c": (self.N,), "f": (19, self.N)}[name]
+        a = np.ascontiguousarray(np.asarray(arr, dtype=np.float32).reshape(shape))
+        _check(load().glsm_fluid_write(self.h, FL_FIELD[name], a.ctypes.data, a.nbytes), f"fluid write {name}")
+
+
+def system_step(gels, coupler: Coupler, fluid: Fluid, it: int):
+    """One time step in sim.cpp's order (sim.cpp:66-81)."""
+    arr = (C.c_void_p * len(gels))(*[g.h for g in gels])
+    _check(load().glsm_system_step(arr, len(gels), coupler.h, fluid.h, int(it)), "glsm_system_step")

@@ -1,0 +1,76 @@
+"""Build recipe for libglsm_fluid.so (gfx950), run in-tree.
+
+``python -m glsm_fluid_3d_amd.build`` compiles every HIP/C++ source under
+csrc/ with hipcc (one object per source, in parallel) and links
+glsm_fluid_3d_amd/libglsm_fluid.so.  Kernels that must follow the
+reference's arithmetic rounding step by step (gel FP64, IBM, scalar,
+coupling) are built with -ffp-contract=off; the LBM kernel keeps FMA
+contraction.  Packed-FP32 instruction selection is disabled for all device
+code.
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+INCLUDE = os.path.join(ROOT, "include")
+BUILD = os.path.join(PKG, "_build")
+LIB = os.path.join(PKG, "libglsm_fluid.so")
+ARCH = os.environ.get("GLSM_OFFLOAD_ARCH", "gfx950")
+
+HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+
+COMMON = [
+    "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-I", INCLUDE, "-I", CSRC, "-Wall", "-Wno-unused-function", "-Wno-unused-result",
+]
+# sources whose arithmetic must be rounded operation by operation
+NO_CONTRACT = {"gel_kernels.hip", "ibm_kernels.hip"}
+SOURCES = ["gel_kernels.hip", "ibm_kernels.hip", "lbm_kernels.hip",
+           "runtime.cpp", "gel.cpp", "coupling.cpp", "fluid.cpp", "capi.cpp"]
+
+
+def _compile(src: str) -> str:
+    obj = os.path.join(BUILD, src + ".o")
+    cmd = [HIPCC] + COMMON
+    if src in NO_CONTRACT:
+        cmd.append("-ffp-contract=off")
+    if src.endswith(".cpp"):
+        cmd += ["-x", "hip"]
+    cmd += ["-c", os.path.join(CSRC, src), "-o", obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed for {src}:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return obj
+
+
+def build(verbose: bool = True) -> str:
+    """Compile all sources and link libglsm_fluid.so; returns its path."""
+    os.makedirs(BUILD, exist_ok=True)
+    jobs = min(len(SOURCES), int(os.environ.get("MAX_JOBS", "8")))
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        objs = list(ex.map(_compile, SOURCES))
+    tmp = LIB + ".tmp"
+    cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs + ["-lpthread"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, LIB)
+    if verbose:
+        print(f"built {LIB}")
+    return LIB
+
+
+if __name__ == "__main__":
+    try:
+        build()
+    except RuntimeError as e:
+        print(e, file=sys.stderr)
+        sys.exit(1)

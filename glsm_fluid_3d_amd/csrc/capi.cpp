@@ -1,0 +1,425 @@
+// capi.cpp -- the C ABI of include/glsm_fluid.h over the Gel / Coupler /
+// Fluid classes.  Every entry point catches C++ exceptions and HIP errors and
+// returns a status; the message is kept per thread for glsm_last_error().
+#include <cstring>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "coupling.h"
+#include "fluid.h"
+#include "gel.h"
+#include "glsm_fluid.h"
+#include "glsm_internal.hpp"
+
+struct glsm_coupler {
+  std::vector<Gel*> gels;  // Coupler keeps a reference to this vector
+  Coupler* c = nullptr;
+};
+
+namespace {
+
+thread_local std::string t_last_error;
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    t_last_error.clear();
+    return f();
+  } catch (const std::invalid_argument& e) {
+    t_last_error = e.what();
+    return GLSM_EINVAL;
+  } catch (const std::bad_alloc& e) {
+    t_last_error = e.what();
+    return GLSM_ENOMEM;
+  } catch (const std::runtime_error& e) {
+    t_last_error = e.what();
+    const std::string m = e.what();
+    if (m.find("failed: hip") != std::string::npos) return GLSM_EDEVICE;
+    if (m.find("missing") != std::string::npos) return GLSM_EIO;
+    return GLSM_ESTATE;
+  } catch (...) {
+    t_last_error = "unknown C++ exception";
+    return GLSM_ESTATE;
+  }
+}
+
+#define REQUIRE(cond, msg)                   \
+  do {                                       \
+    if (!(cond)) throw std::invalid_argument(msg); \
+  } while (0)
+
+Gel* G(glsm_gel* g) {
+  REQUIRE(g, "null gel handle");
+  return reinterpret_cast<Gel*>(g);
+}
+Fluid* F(glsm_fluid* f) {
+  REQUIRE(f, "null fluid handle");
+  return reinterpret_cast<Fluid*>(f);
+}
+glsm_coupler* CP(glsm_coupler* c) {
+  REQUIRE(c && c->c, "null coupler handle");
+  return c;
+}
+
+struct FieldRef {
+  void* dev;
+  size_t bytes;
+};
+
+FieldRef gel_field(Gel* g, int field) {
+  const size_t nE = g->m_numGelElements, nN = g->m_numGelNodes, M = g->m_boundaryCount;
+  const size_t d = sizeof(double), d3 = sizeof(double3), I = sizeof(int);
+  const size_t fil = 1000 * (size_t)g->m_hgp->maxFilamentlen;
+  switch (field) {
+    case GLSM_GEL_UM: return {g->m_dum, nE * d};
+    case GLSM_GEL_VM: return {g->m_dvm, nE * d};
+    case GLSM_GEL_WM: return {g->m_dwm, nE * d};
+    case GLSM_GEL_UM_NORM: return {g->m_dum_norm, nE * d};
+    case GLSM_GEL_VM_NORM: return {g->m_dvm_norm, nE * d};
+    case GLSM_GEL_WMP: return {g->m_dwmp, nE * d};
+    case GLSM_GEL_T0: return {g->m_dT0m, nE * d};
+    case GLSM_GEL_T1: return {g->m_dT1m, nE * d};
+    case GLSM_GEL_T2: return {g->m_dT2m, nE * d};
+    case GLSM_GEL_VOL: return {g->m_dVolm, nE * d};
+    case GLSM_GEL_PM: return {g->m_dPrem, nE * d};
+    case GLSM_GEL_RM: return {g->m_drm, nE * d3};
+    case GLSM_GEL_RM_LOC: return {g->m_drm_loc, 8 * nE * d3};
+    case GLSM_GEL_NMSM: return {g->m_dnmSm, 6 * nE * d3};
+    case GLSM_GEL_RN: return {g->m_drn, nN * d3};
+    case GLSM_GEL_VELN: return {g->m_dVeln, nN * d3};
+    case GLSM_GEL_VELS: return {g->m_dVels, nN * d3};
+    case GLSM_GEL_FN: return {g->m_dFn, nN * d3};
+    case GLSM_GEL_FN_ROBIN: return {g->m_dFn_robin, nN * d3};
+    case GLSM_GEL_FDRAG_ROBIN: return {g->m_dFdrag_robin, nN * d3};
+    case GLSM_GEL_UN_NORM: return {g->m_dun_norm, nN * d};
+    case GLSM_GEL_UN_ROBIN: return {g->m_dun_robin, nN * d};
+    case GLSM_GEL_VN_NORM: return {g->m_dvn_norm, nN * d};
+    case GLSM_GEL_MAP_NODE: return {g->m_dmap_node, nN * I};
+    case GLSM_GEL_MAP_ELEMENT: return {g->m_dmap_element, nE * I};
+    case GLSM_GEL_BINDEX: return {g->m_dbIndex, M * I};
+    case GLSM_GEL_CENTER_VM: return {g->m_dvm_center, 1000 * d};
+    case GLSM_GEL_CENTER_WM: return {g->m_dwm_center, 1000 * d};
+    case GLSM_GEL_CENTER_RM: return {g->m_drm_center, 1000 * d3};
+    case GLSM_GEL_CENTER_FN: return {g->m_dFn_center, 1000 * d3};
+    case GLSM_GEL_CENTER_VELN: return {g->m_dVeln_center, 1000 * d3};
+    case GLSM_GEL_FILAMENT: return {g->m_dfilament, fil * d3};
+    default: throw std::invalid_argument("unknown gel field " + std::to_string(field));
+  }
+}
+
+FieldRef coupler_field(Coupler* c, int field) {
+  const size_t M = c->sumGelBoundaryCount;
+  switch (field) {
+    case GLSM_CP_LAG: return {c->d_lag_all_, M * sizeof(float3)};
+    case GLSM_CP_UL: return {c->d_Ul_all_, M * sizeof(float3)};
+    case GLSM_CP_VL: return {c->d_Vl_all_, M * sizeof(float3)};
+    case GLSM_CP_FDRAG: return {c->d_Fdrag_all_, M * sizeof(float3)};
+    case GLSM_CP_FREP: return {c->d_Frep_all_, M * sizeof(float3)};
+    case GLSM_CP_CL: return {c->d_Cl_all_, M * sizeof(float)};
+    case GLSM_CP_DL: return {c->d_Dl_all_, M * sizeof(float)};
+    case GLSM_CP_SL: return {c->d_Sl_all_, M * sizeof(float)};
+    case GLSM_CP_OWNER: return {c->d_owner, M * sizeof(int)};
+    case GLSM_CP_BINDEX: return {c->d_bIndex_all_, M * sizeof(int)};
+    default: throw std::invalid_argument("unknown coupler field " + std::to_string(field));
+  }
+}
+
+size_t fluid_field_bytes(Fluid* f, int field) {
+  switch (field) {
+    case GLSM_FL_U: return f->N * sizeof(float3);
+    case GLSM_FL_C: return f->N * sizeof(float);
+    case GLSM_FL_F:
+    case GLSM_FL_FPOST: return 19ull * f->N * sizeof(float);
+    default: throw std::invalid_argument("unknown fluid field " + std::to_string(field));
+  }
+}
+
+void d2h(void* host, const void* dev, size_t bytes) {
+  hipStream_t s = glsm::stream();
+  GLSM_HIP(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, s));
+  GLSM_HIP(hipStreamSynchronize(s));
+}
+void h2d(void* dev, const void* host, size_t bytes) {
+  hipStream_t s = glsm::stream();
+  GLSM_HIP(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, s));
+  GLSM_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace
+
+extern "C" {
+
+int glsm_init(int device) {
+  return guarded([&] {
+    REQUIRE(device >= 0, "device must be >= 0");
+    glsm::set_device(device);
+    (void)glsm::stream();
+    return GLSM_OK;
+  });
+}
+
+int glsm_synchronize(void) {
+  return guarded([&] {
+    GLSM_HIP(hipDeviceSynchronize());
+    return GLSM_OK;
+  });
+}
+
+const char* glsm_last_error(void) { return t_last_error.c_str(); }
+const char* glsm_version(void) { return "glsm_fluid_3d_amd 0.1 (gfx950)"; }
+
+// ---------------------------------------------------------------- gel
+int glsm_gel_create(int sx, int sy, int sz, double px, double py, double pz, int gel_type, int gel_id, int time,
+                    glsm_gel** out) {
+  return guarded([&] {
+    REQUIRE(out, "out is null");
+    REQUIRE(sx >= 1 && sy >= 1 && sz >= 1, "gel size must be >= 1 element per axis");
+    *out = reinterpret_cast<glsm_gel*>(
+        new Gel(make_int3(sx, sy, sz), make_double3(px, py, pz), gel_type, gel_id, time));
+    return GLSM_OK;
+  });
+}
+int glsm_gel_destroy(glsm_gel* g) {
+  return guarded([&] {
+    delete G(g);
+    return GLSM_OK;
+  });
+}
+int glsm_gel_step_elasticity(glsm_gel* g, int iter) {
+  return guarded([&] { G(g)->stepElasticity(iter); return GLSM_OK; });
+}
+int glsm_gel_step_chemistry(glsm_gel* g, int iter) {
+  return guarded([&] { G(g)->stepChemistry(iter); return GLSM_OK; });
+}
+int glsm_gel_record_center(glsm_gel* g, int iter) {
+  return guarded([&] { G(g)->recordCenter(iter); return GLSM_OK; });
+}
+int glsm_gel_write_files(glsm_gel* g, int iter) {
+  return guarded([&] { G(g)->writeFiles(iter); return GLSM_OK; });
+}
+int glsm_gel_reset_to_quiescent(glsm_gel* g) {
+  return guarded([&] { G(g)->resetToQuiescent(); return GLSM_OK; });
+}
+int glsm_gel_fire_excitation_pulse(glsm_gel* g) {
+  return guarded([&] { G(g)->fireExcitationPulse(); return GLSM_OK; });
+}
+int glsm_gel_set_anchor_z(glsm_gel* g, double z) {
+  return guarded([&] { G(g)->setAnchorZ(z); return GLSM_OK; });
+}
+int glsm_gel_build_tube_mask(glsm_gel* g, int wall, int circular) {
+  return guarded([&] { G(g)->buildTubeMask(wall, circular != 0); return GLSM_OK; });
+}
+int glsm_gel_get_index(glsm_gel* g, int xi, int yi, int zi, int size) {
+  int r = 0;
+  int rc = guarded([&] { r = G(g)->get_index(xi, yi, zi, size); return GLSM_OK; });
+  return rc == GLSM_OK ? r : rc;
+}
+int glsm_gel_dims(glsm_gel* g, int grid[3], int* nN, int* nE, int* M) {
+  return guarded([&] {
+    Gel* p = G(g);
+    REQUIRE(grid && nN && nE && M, "null output pointer");
+    grid[0] = p->m_gelNodeGrid.x;
+    grid[1] = p->m_gelNodeGrid.y;
+    grid[2] = p->m_gelNodeGrid.z;
+    *nN = p->m_numGelNodes;
+    *nE = p->m_numGelElements;
+    *M = p->m_boundaryCount;
+    return GLSM_OK;
+  });
+}
+int glsm_gel_perturb_chemistry(glsm_gel* g, uint64_t seed, double amplitude) {
+  return guarded([&] { G(g)->perturbChemistry(seed, amplitude); return GLSM_OK; });
+}
+size_t glsm_gel_field_bytes(glsm_gel* g, int field) {
+  size_t b = 0;
+  guarded([&] { b = gel_field(G(g), field).bytes; return GLSM_OK; });
+  return b;
+}
+int glsm_gel_read(glsm_gel* g, int field, void* host, size_t bytes) {
+  return guarded([&] {
+    const FieldRef r = gel_field(G(g), field);
+    REQUIRE(host && bytes == r.bytes, "gel read: size mismatch");
+    d2h(host, r.dev, bytes);
+    return GLSM_OK;
+  });
+}
+int glsm_gel_write(glsm_gel* g, int field, const void* host, size_t bytes) {
+  return guarded([&] {
+    Gel* p = G(g);
+    const FieldRef r = gel_field(p, field);
+    REQUIRE(host && bytes == r.bytes, "gel write: size mismatch");
+    h2d(r.dev, host, bytes);
+    // keep the host chemistry mirrors in step (the phase controls upload them)
+    if (field == GLSM_GEL_UM) std::memcpy(p->m_hum, host, bytes);
+    if (field == GLSM_GEL_VM) std::memcpy(p->m_hvm, host, bytes);
+    if (field == GLSM_GEL_WM) std::memcpy(p->m_hwm, host, bytes);
+    if (field == GLSM_GEL_RN) p->copyDataToHost();
+    return GLSM_OK;
+  });
+}
+
+// ---------------------------------------------------------------- coupler
+int glsm_coupler_create(glsm_gel* const* gels, int n, glsm_coupler** out) {
+  return guarded([&] {
+    REQUIRE(out && (gels || n == 0) && n >= 0, "bad coupler arguments");
+    auto* h = new glsm_coupler;
+    try {
+      for (int i = 0; i < n; ++i) h->gels.push_back(G(gels[i]));
+      h->c = new Coupler(h->gels);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+    return GLSM_OK;
+  });
+}
+int glsm_coupler_destroy(glsm_coupler* c) {
+  return guarded([&] {
+    delete CP(c)->c;
+    delete c;
+    return GLSM_OK;
+  });
+}
+int glsm_coupler_pack_from_gels(glsm_coupler* c) {
+  return guarded([&] { CP(c)->c->packFromGels(); return GLSM_OK; });
+}
+int glsm_coupler_scatter_to_gels(glsm_coupler* c) {
+  return guarded([&] { CP(c)->c->scatterToGels(); return GLSM_OK; });
+}
+int glsm_coupler_apply_gel_repulsion(glsm_coupler* c) {
+  return guarded([&] { CP(c)->c->applyGelRepulsion(); return GLSM_OK; });
+}
+int glsm_coupler_marker_count(glsm_coupler* c) {
+  int m = 0;
+  int rc = guarded([&] { m = CP(c)->c->sumGelBoundaryCount; return GLSM_OK; });
+  return rc == GLSM_OK ? m : rc;
+}
+size_t glsm_coupler_field_bytes(glsm_coupler* c, int field) {
+  size_t b = 0;
+  guarded([&] { b = coupler_field(CP(c)->c, field).bytes; return GLSM_OK; });
+  return b;
+}
+int glsm_coupler_read(glsm_coupler* c, int field, void* host, size_t bytes) {
+  return guarded([&] {
+    const FieldRef r = coupler_field(CP(c)->c, field);
+    REQUIRE(host && bytes == r.bytes, "coupler read: size mismatch");
+    d2h(host, r.dev, bytes);
+    return GLSM_OK;
+  });
+}
+
+// ---------------------------------------------------------------- fluid
+int glsm_fluid_create(int sx, int sy, int sz, int time, glsm_coupler* c, glsm_fluid** out) {
+  return guarded([&] {
+    REQUIRE(out, "out is null");
+    REQUIRE(sx >= 1 && sy >= 1 && sz >= 1, "fluid size must be >= 1 per axis");
+    *out = reinterpret_cast<glsm_fluid*>(new Fluid(make_int3(sx, sy, sz), time, CP(c)->c));
+    return GLSM_OK;
+  });
+}
+int glsm_fluid_destroy(glsm_fluid* f) {
+  return guarded([&] {
+    delete F(f);
+    return GLSM_OK;
+  });
+}
+int glsm_fluid_step_velocity(glsm_fluid* f, int iter) {
+  return guarded([&] { F(f)->stepVelocity(iter); return GLSM_OK; });
+}
+int glsm_fluid_sub_step(glsm_fluid* f, float beta_eff) {
+  return guarded([&] { F(f)->subStep(beta_eff, true); return GLSM_OK; });
+}
+int glsm_fluid_step_concentration(glsm_fluid* f, int iter) {
+  return guarded([&] { F(f)->stepConcentration(iter); return GLSM_OK; });
+}
+int glsm_fluid_write_files(glsm_fluid* f, int iter) {
+  return guarded([&] { F(f)->writeFiles(iter); return GLSM_OK; });
+}
+int glsm_fluid_set_periodic(glsm_fluid* f, int px, int py, int pz) {
+  return guarded([&] { F(f)->setPeriodic(px, py, pz); return GLSM_OK; });
+}
+int glsm_fluid_set_ibm(glsm_fluid* f, float beta, float ramp) {
+  return guarded([&] { F(f)->setIBM(beta, ramp); return GLSM_OK; });
+}
+int glsm_fluid_set_storage(glsm_fluid* f, int storage, int scale_exp) {
+  return guarded([&] { F(f)->setStorage(storage, scale_exp); return GLSM_OK; });
+}
+int glsm_fluid_enable_lbm_timing(glsm_fluid* f, int every, int max_samples) {
+  return guarded([&] { F(f)->enableLbmTiming(every, max_samples); return GLSM_OK; });
+}
+int glsm_fluid_lbm_timing(glsm_fluid* f, double* mean_ms, int* samples) {
+  return guarded([&] {
+    REQUIRE(mean_ms, "null output pointer");
+    *mean_ms = F(f)->lbmKernelTiming(samples);
+    return GLSM_OK;
+  });
+}
+int glsm_fluid_dims(glsm_fluid* f, int grid[3], int* nsub) {
+  return guarded([&] {
+    Fluid* p = F(f);
+    REQUIRE(grid && nsub, "null output pointer");
+    grid[0] = p->fluidNodeGrid.x;
+    grid[1] = p->fluidNodeGrid.y;
+    grid[2] = p->fluidNodeGrid.z;
+    *nsub = p->Nsub;
+    return GLSM_OK;
+  });
+}
+size_t glsm_fluid_field_bytes(glsm_fluid* f, int field) {
+  size_t b = 0;
+  guarded([&] { b = fluid_field_bytes(F(f), field); return GLSM_OK; });
+  return b;
+}
+int glsm_fluid_read(glsm_fluid* f, int field, void* host, size_t bytes) {
+  return guarded([&] {
+    Fluid* p = F(f);
+    REQUIRE(host && bytes == fluid_field_bytes(p, field), "fluid read: size mismatch");
+    switch (field) {
+      case GLSM_FL_U: d2h(host, p->d_u, bytes); break;
+      case GLSM_FL_C: d2h(host, p->d_c1, bytes); break;
+      case GLSM_FL_F: p->readPopulations(static_cast<float*>(host)); break;
+      case GLSM_FL_FPOST: p->readPostCollision(static_cast<float*>(host)); break;
+    }
+    return GLSM_OK;
+  });
+}
+int glsm_fluid_write(glsm_fluid* f, int field, const void* host, size_t bytes) {
+  return guarded([&] {
+    Fluid* p = F(f);
+    REQUIRE(host && bytes == fluid_field_bytes(p, field), "fluid write: size mismatch");
+    switch (field) {
+      case GLSM_FL_U: h2d(p->d_u, host, bytes); break;
+      case GLSM_FL_C: h2d(p->d_c1, host, bytes); break;
+      case GLSM_FL_F: p->writePopulations(static_cast<const float*>(host)); break;
+      default: throw std::invalid_argument("fluid field is read-only");
+    }
+    return GLSM_OK;
+  });
+}
+
+// ---------------------------------------------------------------- system
+int glsm_system_step(glsm_gel* const* gels, int n, glsm_coupler* c, glsm_fluid* f, int iter) {
+  return guarded([&] {
+    glsm_coupler* cp = CP(c);
+    Fluid* fl = F(f);
+    REQUIRE((int)cp->gels.size() == n, "gel count differs from the coupler's");
+    for (int i = 0; i < n; ++i) REQUIRE(G(gels[i]) == cp->gels[i], "gel order differs from the coupler's");
+    for (int i = 0; i < n; ++i) {
+      Gel* g = cp->gels[i];
+      g->stepElasticity(iter);
+      g->stepChemistry(iter);
+      g->recordCenter(iter);
+    }
+    cp->c->packFromGels();
+    fl->stepVelocity(iter);
+    fl->stepConcentration(iter);
+    cp->c->applyGelRepulsion();
+    cp->c->scatterToGels();
+    return GLSM_OK;
+  });
+}
+
+}  // extern "C"

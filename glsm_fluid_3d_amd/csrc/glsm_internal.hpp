@@ -1,0 +1,166 @@
+// glsm_internal.hpp -- device-side views, launch entry points and runtime
+// helpers shared by the library's translation units.  Not installed.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+namespace glsm {
+
+// ---- runtime ----------------------------------------------------------------
+hipStream_t stream();  // the one library stream (created on first use)
+void set_device(int device);
+
+[[noreturn]] void throw_hip(hipError_t e, const char* what, const char* file, int line);
+#define GLSM_HIP(call)                                              \
+  do {                                                              \
+    hipError_t _e = (call);                                         \
+    if (_e != hipSuccess) ::glsm::throw_hip(_e, #call, __FILE__, __LINE__); \
+  } while (0)
+#define GLSM_LAUNCH_CHECK() GLSM_HIP(hipGetLastError())
+
+template <class T>
+T* dalloc(size_t n) {  // zero-filled device allocation
+  void* p = nullptr;
+  if (n == 0) n = 1;
+  GLSM_HIP(hipMalloc(&p, n * sizeof(T)));
+  GLSM_HIP(hipMemset(p, 0, n * sizeof(T)));
+  return static_cast<T*>(p);
+}
+inline void dfree(void* p) {
+  if (p) (void)hipFree(p);
+}
+
+// ---- gel --------------------------------------------------------------------
+// Everything a gel kernel reads: dimensions, constants, the 27-entry offset
+// table (ghost mirror / no-flux copy direction per type code) and the
+// device arrays (reference layouts).
+struct GelView {
+  int LX, LY, LZ;
+  int gelType;
+  double f, ep, dt, dtx, dx, dy, dz;
+  double CH0, CH1, CHS, C0, AZ0, FA0, c0_bis, b;
+  double uss, vss, wss;
+  int tube_mode;
+  float tube_cy, tube_cz, tube_inner_hy, tube_inner_hz;
+  int use_anchor;
+  double anchor_z;
+  int maxFilamentlen;
+  int off[27][3];
+  double *um, *vm, *wm, *um_norm, *vm_norm, *wmp, *T0, *T1, *T2, *vol, *pm;
+  double3 *rm, *rm_loc, *nmSm;
+  double3 *rn, *Veln, *Vels, *Fn, *Fn_robin, *Fdrag_robin;
+  double *un_norm, *un_robin, *vn_norm;
+  const int *map_node, *map_element;
+  double *vm_center, *wm_center;
+  double3 *rm_center, *Fn_center, *Veln_center, *filament;
+};
+
+void launch_gel_elasticity(const GelView& g, hipStream_t s);  // iter % 5 == 0 block
+void launch_gel_set_zero(const GelView& g, hipStream_t s);
+void launch_gel_chemistry(const GelView& g, hipStream_t s);
+void launch_gel_record_center(const GelView& g, int slot, hipStream_t s);
+
+// ---- coupler ----------------------------------------------------------------
+}  // namespace glsm
+
+// per-gel pointers used by the marker pack / scatter kernels
+struct GelMarkerDesc {
+  const double3* rn;
+  const double3* Vels;
+  const double* un_norm;
+  double* un_robin;
+  double3* Fn_robin;
+  double3* Fdrag_robin;
+};
+
+// one (marker, weight) contribution to a lattice cell; entries of a cell are
+// stored in increasing marker order
+struct SpreadEntry {
+  int marker;
+  float w;
+};
+
+namespace glsm {
+
+struct MarkerArrays {
+  int M;
+  const int* owner;
+  const int* bIndex;  // node index inside the owner gel
+  const GelMarkerDesc* desc;
+  float3 *lag, *Ul, *Vl, *Fdrag, *Frep;
+  float *Cl, *Dl, *Sl;
+};
+
+}  // namespace glsm
+class Coupler;
+namespace glsm {
+MarkerArrays markers_of(Coupler& c);  // lattice-side view (no gel table)
+
+void launch_pack(const MarkerArrays& m, float gel_to_lbm_vel, hipStream_t s);
+void launch_scatter(const MarkerArrays& m, hipStream_t s);
+void launch_repulsion(const MarkerArrays& m, float delta, float gamma, hipStream_t s);
+
+// ---- lattice ------------------------------------------------------------------
+struct LatticeGeom {
+  int Nx, Ny, Nz, N;
+  int perX, perY, perZ;
+  float h;
+  double rNx, rNxNy;  // 1/Nx, 1/(Nx*Ny) for exact index decoding
+};
+
+struct SpreadLists {
+  int nblk;
+  int* blkflag;
+  int* cnt;
+  int* fillpos;
+  int2* rng;
+  int* bsum;  // nblk + 1
+  SpreadEntry* ent;
+  int* ctab;    // 64 per marker: stencil cell or -1 (rebuild scratch)
+  float* wtab;  // 64 per marker: spreading weight (rebuild scratch)
+};
+
+// rebuild the per-cell contribution lists for the current marker positions
+void launch_spread_rebuild(const LatticeGeom& g, const MarkerArrays& m, const SpreadLists& sl,
+                           hipStream_t s);
+// Ul = interp(u), Fdrag = beta_eff (Vl - Ul)
+void launch_ibm_interp(const LatticeGeom& g, const MarkerArrays& m, const float3* u, float beta_eff,
+                       hipStream_t s);
+// Dl = interp(c), Sl = Cl - Dl
+void launch_ibm_sample(const LatticeGeom& g, const MarkerArrays& m, const float* c, hipStream_t s);
+
+// distribution storage representations (collision is FP32 in all of them)
+enum StorageMode { F32_FULL = 0, F32_DEV = 1, F16_DEV = 2, BF16_DEV = 3 };
+size_t storage_bytes(int storage);
+
+struct LbmParams {
+  float omega;     // 1/tau
+  float w0, w1, w2;
+  float3 Fc;       // FluidParams::F_const
+  float scale;     // FP16 deviation scale 2^s (1 otherwise)
+  float inv_scale;
+};
+// one fused pull stream + collide sub-step: fin -> fout (post-collision)
+void launch_lbm_step(int storage, const LatticeGeom& g, const LbmParams& p, const void* fin, void* fout,
+                     size_t plane, float3* u, bool write_u_all, const SpreadLists& sl, const float3* Fl,
+                     hipStream_t s);
+void launch_lbm_init(int storage, const LatticeGeom& g, const LbmParams& p, void* f, size_t plane,
+                     float3* u, float* c1, float* c2, hipStream_t s);
+// stored state -> FP32 full populations (pull = pre-collision of the next
+// sub-step, else the stored post-collision ones), and the inverse of pull
+void launch_lbm_export(int storage, bool pull, const LatticeGeom& g, const LbmParams& p, const void* fpost,
+                       float* out, size_t plane, size_t out_plane, hipStream_t s);
+void launch_lbm_import(int storage, const LatticeGeom& g, const LbmParams& p, const float* in, void* fpost,
+                       size_t plane, size_t in_plane, hipStream_t s);
+
+struct ScalarParams {
+  float dt, D, dx, dy, dz;
+};
+void launch_scalar(const LatticeGeom& g, const ScalarParams& p, const float3* u, const float* c1,
+                   float* c2, const SpreadLists& sl, const float* Sl, hipStream_t s);
+
+}  // namespace glsm

@@ -1,0 +1,457 @@
+// ibm_kernels.hip -- marker pack/scatter/repulsion, immersed-boundary
+// interpolation and spreading lists, and the concentration step (gfx950).
+//
+// Built with -ffp-contract=off.  Every sum runs in the reference's loop
+// order (stencil x outer, z inner; markers in increasing index), so with the
+// same inputs these kernels reproduce the FP32 restatement bit for bit.
+//
+// Design (vs. the reference's one-thread-per-marker loops with atomicAdd,
+// fluid_kernels.cu:244-459):
+//   * interpolation: one 64-lane wavefront per marker, one lane per cell of
+//     the 4x4x4 Peskin stencil; the 64 products are summed in stencil order
+//     by lane 0 from LDS, which keeps the reference's summation order;
+//   * spreading: gathered, not scattered.  When the markers move, per-cell
+//     lists of (marker, weight) are rebuilt (count, block scan, fill, sort by
+//     marker).  The fused LBM kernel and the scalar kernel then sum a cell's
+//     contributions in marker order -- no atomics, the same bits every run.
+#include "glsm_internal.hpp"
+
+namespace glsm {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kWavesPerBlock = kThreads / 64;
+constexpr int kFlagCells = 256;  // cells per spreading block flag
+
+inline int blocks_for(long n, int per) { return (int)((n + per - 1) / per); }
+
+// Peskin 4-point kernel, fluid_kernels.cu:60-71
+__device__ inline float delta4(float r) {
+  r = fabsf(r);
+  if (r <= 1.f) return 0.125f * (3.f - 2.f * r + sqrtf(1.f + 4.f * r - 4.f * r * r));
+  if (r <= 2.f) return 0.125f * (5.f - 2.f * r - sqrtf(-7.f + 12.f * r - 4.f * r * r));
+  return 0.f;
+}
+
+// One lane's stencil cell of marker position p: cell index (or -1 when the
+// cell is clipped by a wall) and the weight phi_x*phi_y*phi_z.
+struct StencilLane {
+  int cell;
+  float w;
+};
+__device__ inline int axis_cell(int i, int n, int per, bool& ok) {
+  if (per) {
+    ok = true;
+    return ((i % n) + n) % n;
+  }
+  ok = (i >= 0 && i < n);
+  return i;
+}
+__device__ inline StencilLane stencil_lane(const LatticeGeom& g, float3 p, int lane) {
+  const float gx = p.x / g.h, gy = p.y / g.h, gz = p.z / g.h;
+  const int ix = (int)floorf(gx), iy = (int)floorf(gy), iz = (int)floorf(gz);
+  const int ii = ix - 1 + (lane >> 4), jj = iy - 1 + ((lane >> 2) & 3), kk = iz - 1 + (lane & 3);
+  bool okx, oky, okz;
+  const int cx = axis_cell(ii, g.Nx, g.perX, okx);
+  const int cy = axis_cell(jj, g.Ny, g.perY, oky);
+  const int cz = axis_cell(kk, g.Nz, g.perZ, okz);
+  StencilLane s;
+  if (okx && oky && okz) {
+    const float phix = delta4(gx - (float)ii), phiy = delta4(gy - (float)jj), phiz = delta4(gz - (float)kk);
+    s.w = phix * phiy * phiz;
+    s.cell = cx + cy * g.Nx + cz * g.Nx * g.Ny;
+  } else {
+    s.w = 0.f;
+    s.cell = -1;
+  }
+  return s;
+}
+
+// ---------------------------------------------------------------- coupler
+// k_gather_boundary, coupling_kernels.cu:42-59, all gels in one launch
+__global__ void __launch_bounds__(kThreads) k_pack(MarkerArrays m, float g2l) {
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= m.M) return;
+  const GelMarkerDesc d = m.desc[m.owner[l]];
+  const int id = m.bIndex[l];
+  const double3 r = d.rn[id], v = d.Vels[id];
+  m.lag[l] = make_float3((float)r.x, (float)r.y, (float)r.z);
+  const double s = (double)g2l;
+  m.Vl[l] = make_float3((float)(v.x * s), (float)(v.y * s), (float)(v.z * s));
+  m.Cl[l] = (float)d.un_norm[id];
+}
+
+// k_add_reaction_to_gel + k_add_drag_to_gel, coupling_kernels.cu:8-34;
+// every boundary node owns exactly one marker, so no two threads collide
+__global__ void __launch_bounds__(kThreads) k_scatter(MarkerArrays m) {
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= m.M) return;
+  const GelMarkerDesc d = m.desc[m.owner[l]];
+  const int id = m.bIndex[l];
+  const float3 fr = m.Frep[l], fd = m.Fdrag[l];
+  const float s = m.Sl[l];
+  double3 a = d.Fn_robin[id];
+  a.x -= (double)fr.x; a.y -= (double)fr.y; a.z -= (double)fr.z;
+  d.Fn_robin[id] = a;
+  d.un_robin[id] -= (double)s;
+  double3 b = d.Fdrag_robin[id];
+  b.x -= (double)fd.x; b.y -= (double)fd.y; b.z -= (double)fd.z;
+  d.Fdrag_robin[id] = b;
+}
+
+// Lennard-Jones repulsion between markers of different gels
+// (k_gel_repulsion / bodyBodyInteraction, coupling_kernels.cu:61-146):
+// positions staged through LDS tiles, partners visited in index order, the
+// result written once per marker (no atomics).
+__global__ void __launch_bounds__(kThreads) k_repulsion(MarkerArrays m, float delta, float gamma) {
+  __shared__ float4 s_pos[kThreads];
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  const float rcut = 2.5f * gamma, rc2 = rcut * rcut, sig2 = gamma * gamma;
+  float3 pa = make_float3(0.f, 0.f, 0.f);
+  int oa = -1;
+  if (a < m.M) {
+    pa = m.lag[a];
+    oa = m.owner[a];
+  }
+  float fx = 0.f, fy = 0.f, fz = 0.f;
+  for (int base = 0; base < m.M; base += kThreads) {
+    const int b = base + threadIdx.x;
+    if (b < m.M) {
+      const float3 pb = m.lag[b];
+      s_pos[threadIdx.x] = make_float4(pb.x, pb.y, pb.z, __int_as_float(m.owner[b]));
+    } else {
+      s_pos[threadIdx.x] = make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
+    }
+    __syncthreads();
+    if (a < m.M) {
+      const int nb = min(kThreads, m.M - base);
+      for (int k = 0; k < nb; ++k) {
+        const float4 q = s_pos[k];
+        if (__float_as_int(q.w) == oa) continue;
+        const float dx = pa.x - q.x, dy = pa.y - q.y, dz = pa.z - q.z;
+        const float d2 = dx * dx + dy * dy + dz * dz;
+        if (d2 <= 0.0f || d2 >= rc2) continue;
+        const float inv = 1.0f / d2;
+        const float sr2 = sig2 * inv;
+        const float sr6 = sr2 * sr2 * sr2;
+        const float sr12 = sr6 * sr6;
+        const float fac = 24.0f * delta * (2.0f * sr12 - sr6) * inv;
+        fx += fac * dx;
+        fy += fac * dy;
+        fz += fac * dz;
+      }
+    }
+    __syncthreads();
+  }
+  if (a < m.M) m.Frep[a] = make_float3(0.f + fx, 0.f + fy, 0.f + fz);
+}
+
+// ---------------------------------------------------------------- IBM
+// Ul = sum(u*phi)/sum(phi) over the stencil, Fdrag = beta_eff*(Vl - Ul)
+// (k_ibm_interpolate_velocity + k_scale_negbeta, fluid_kernels.cu:244-291)
+__global__ void __launch_bounds__(kThreads) k_ibm_interp(LatticeGeom g, MarkerArrays m, const float3* __restrict__ u,
+                                                         float beta_eff) {
+  __shared__ float4 s[kWavesPerBlock][64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int l = blockIdx.x * kWavesPerBlock + wv;
+  const bool live = l < m.M;
+  float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (live) {
+    const StencilLane st = stencil_lane(g, m.lag[l], lane);
+    if (st.cell >= 0) {
+      const float3 uc = u[st.cell];
+      v = make_float4(uc.x * st.w, uc.y * st.w, uc.z * st.w, st.w);
+    }
+  }
+  s[wv][lane] = v;
+  __syncthreads();
+  if (live && lane == 0) {
+    float ux = 0.f, uy = 0.f, uz = 0.f, ws = 0.f;
+    for (int k = 0; k < 64; ++k) {
+      const float4 t = s[wv][k];
+      ux += t.x; uy += t.y; uz += t.z; ws += t.w;
+    }
+    float3 ul = make_float3(0.f, 0.f, 0.f);
+    if (ws > 0.0f) ul = make_float3(ux / ws, uy / ws, uz / ws);
+    m.Ul[l] = ul;
+    const float3 vl = m.Vl[l];
+    m.Fdrag[l] = make_float3(beta_eff * (vl.x - ul.x), beta_eff * (vl.y - ul.y), beta_eff * (vl.z - ul.z));
+  }
+}
+
+// Dl = sum(c*phi)/sum(phi), Sl = Cl - Dl (k_ibm_sample_concentration +
+// k_robin_boundary, fluid_kernels.cu:350-387, 293-298)
+__global__ void __launch_bounds__(kThreads) k_ibm_sample(LatticeGeom g, MarkerArrays m, const float* __restrict__ c) {
+  __shared__ float2 s[kWavesPerBlock][64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int l = blockIdx.x * kWavesPerBlock + wv;
+  const bool live = l < m.M;
+  float2 v = make_float2(0.f, 0.f);
+  if (live) {
+    const StencilLane st = stencil_lane(g, m.lag[l], lane);
+    if (st.cell >= 0) v = make_float2(c[st.cell] * st.w, st.w);
+  }
+  s[wv][lane] = v;
+  __syncthreads();
+  if (live && lane == 0) {
+    float cl = 0.f, ws = 0.f;
+    for (int k = 0; k < 64; ++k) {
+      cl += s[wv][k].x;
+      ws += s[wv][k].y;
+    }
+    const float dl = ws > 0.0f ? cl / ws : 0.0f;
+    m.Dl[l] = dl;
+    m.Sl[l] = 1 * (m.Cl[l] - dl);
+  }
+}
+
+// ---- spreading lists ----
+// Per marker: the spreading weight of each stencil cell, phi*(1/h^3)/sum(phi)
+// (k_ibm_spread_forces, fluid_kernels.cu:300-348); a marker whose weights
+// sum to zero spreads nothing.  Counts contributions per cell and flags the
+// 256-cell blocks that receive any.
+__global__ void __launch_bounds__(kThreads) k_spread_count(LatticeGeom g, MarkerArrays m, SpreadLists sl,
+                                                           int* __restrict__ ctab, float* __restrict__ wtab) {
+  __shared__ float s[kWavesPerBlock][64];
+  __shared__ float s_scale[kWavesPerBlock];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int l = blockIdx.x * kWavesPerBlock + wv;
+  const bool live = l < m.M;
+  StencilLane st{-1, 0.f};
+  if (live) st = stencil_lane(g, m.lag[l], lane);
+  s[wv][lane] = st.w;
+  __syncthreads();
+  if (lane == 0) {
+    float ws = 0.f;
+    for (int k = 0; k < 64; ++k) ws += s[wv][k];
+    const float inv_h3 = (float)(1.0 / (double)(g.h * g.h * g.h));
+    s_scale[wv] = ws == 0.0f ? 0.0f : 1 * inv_h3 / ws;
+    if (ws == 0.0f) s_scale[wv] = -1.0f;  // marker spreads nothing
+  }
+  __syncthreads();
+  if (!live) return;
+  const float scale = s_scale[wv];
+  const size_t k = (size_t)l * 64 + lane;
+  if (st.cell < 0 || scale < 0.0f) {
+    ctab[k] = -1;
+    return;
+  }
+  ctab[k] = st.cell;
+  wtab[k] = st.w * scale;
+  atomicAdd(&sl.cnt[st.cell], 1);
+  sl.blkflag[st.cell / kFlagCells] = 1;
+}
+
+__global__ void k_spread_clear_flags(SpreadLists sl) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < sl.nblk) sl.blkflag[b] = 0;
+}
+
+__device__ inline int block_excl_scan(int v, int* s_tmp, int& total) {
+  // exclusive scan over kThreads values (Hillis-Steele in LDS)
+  const int t = threadIdx.x;
+  s_tmp[t] = v;
+  __syncthreads();
+  for (int o = 1; o < kThreads; o <<= 1) {
+    const int a = t >= o ? s_tmp[t - o] : 0;
+    __syncthreads();
+    s_tmp[t] += a;
+    __syncthreads();
+  }
+  total = s_tmp[kThreads - 1];
+  const int incl = s_tmp[t];
+  __syncthreads();
+  return incl - v;
+}
+
+// per flagged block: total contributions
+__global__ void __launch_bounds__(kThreads) k_spread_block_sum(LatticeGeom g, SpreadLists sl) {
+  __shared__ int s_tmp[kThreads];
+  const int b = blockIdx.x;
+  if (!sl.blkflag[b]) {
+    if (threadIdx.x == 0) sl.bsum[b] = 0;
+    return;
+  }
+  const int i = b * kFlagCells + threadIdx.x;
+  const int v = i < g.N ? sl.cnt[i] : 0;
+  int total;
+  block_excl_scan(v, s_tmp, total);
+  if (threadIdx.x == 0) sl.bsum[b] = total;
+}
+
+// exclusive scan of the block totals (one work-group); bsum[nblk] = total
+__global__ void __launch_bounds__(kThreads) k_spread_scan_blocks(SpreadLists sl) {
+  __shared__ int s_tmp[kThreads];
+  __shared__ int s_carry;
+  if (threadIdx.x == 0) s_carry = 0;
+  __syncthreads();
+  for (int base = 0; base < sl.nblk; base += kThreads) {
+    const int b = base + threadIdx.x;
+    const int v = b < sl.nblk ? sl.bsum[b] : 0;
+    int total;
+    const int ex = block_excl_scan(v, s_tmp, total);
+    const int carry = s_carry;
+    if (b < sl.nblk) sl.bsum[b] = carry + ex;
+    __syncthreads();
+    if (threadIdx.x == 0) s_carry = carry + total;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) sl.bsum[sl.nblk] = s_carry;
+}
+
+// per flagged block: each cell's first entry and count; counts reset to 0
+__global__ void __launch_bounds__(kThreads) k_spread_local(LatticeGeom g, SpreadLists sl) {
+  __shared__ int s_tmp[kThreads];
+  const int b = blockIdx.x;
+  if (!sl.blkflag[b]) return;
+  const int i = b * kFlagCells + threadIdx.x;
+  const int v = i < g.N ? sl.cnt[i] : 0;
+  int total;
+  const int ex = block_excl_scan(v, s_tmp, total);
+  if (i < g.N) {
+    sl.rng[i] = make_int2(sl.bsum[b] + ex, v);
+    sl.cnt[i] = 0;
+  }
+}
+
+__global__ void __launch_bounds__(kThreads) k_spread_fill(int M64, SpreadLists sl, const int* __restrict__ ctab,
+                                                          const float* __restrict__ wtab) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= M64) return;
+  const int cell = ctab[k];
+  if (cell < 0) return;
+  const int pos = sl.rng[cell].x + atomicAdd(&sl.fillpos[cell], 1);
+  SpreadEntry e;
+  e.marker = k >> 6;
+  e.w = wtab[k];
+  sl.ent[pos] = e;
+}
+
+// order each cell's entries by marker (insertion sort; a few dozen at most)
+__global__ void __launch_bounds__(kThreads) k_spread_sort(LatticeGeom g, SpreadLists sl) {
+  const int b = blockIdx.x;
+  if (!sl.blkflag[b]) return;
+  const int i = b * kFlagCells + threadIdx.x;
+  if (i >= g.N) return;
+  const int2 r = sl.rng[i];
+  sl.fillpos[i] = 0;
+  SpreadEntry* e = sl.ent + r.x;
+  for (int a = 1; a < r.y; ++a) {
+    const SpreadEntry key = e[a];
+    int j = a - 1;
+    while (j >= 0 && e[j].marker > key.marker) {
+      e[j + 1] = e[j];
+      --j;
+    }
+    e[j + 1] = key;
+  }
+}
+
+// ---------------------------------------------------------------- scalar
+__device__ inline void decode_cell(const LatticeGeom& g, int i, int& x, int& y, int& z) {
+  // exact i / (Nx*Ny) and i / Nx via a double reciprocal and one correction
+  int q = (int)((double)i * g.rNxNy);
+  int r = i - q * g.Nx * g.Ny;
+  if (r < 0) { --q; r += g.Nx * g.Ny; } else if (r >= g.Nx * g.Ny) { ++q; r -= g.Nx * g.Ny; }
+  z = q;
+  int q2 = (int)((double)r * g.rNx);
+  int r2 = r - q2 * g.Nx;
+  if (r2 < 0) { --q2; r2 += g.Nx; } else if (r2 >= g.Nx) { ++q2; r2 -= g.Nx; }
+  y = q2;
+  x = r2;
+}
+
+// first-order upwind advection + 7-point diffusion + spread IBM source,
+// forward Euler (k_zero_scalar + k_ibm_spread_concentration +
+// k_convection_diffusion, fluid_kernels.cu:119-123, 414-459, 177-242)
+__global__ void __launch_bounds__(kThreads) k_scalar(LatticeGeom g, ScalarParams p, const float3* __restrict__ u,
+                                                     const float* __restrict__ c1, float* __restrict__ c2,
+                                                     SpreadLists sl, const float* __restrict__ Sl) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int flagged = sl.blkflag[blockIdx.x];
+  if (i >= g.N) return;
+  int x, y, z;
+  decode_cell(g, i, x, y, z);
+  const int Nx = g.Nx, Ny = g.Ny, Nz = g.Nz;
+  const int xp = (x + 1 < Nx) ? x + 1 : (g.perX ? 0 : x);
+  const int xm = (x - 1 >= 0) ? x - 1 : (g.perX ? Nx - 1 : x);
+  const int yp = (y + 1 < Ny) ? y + 1 : (g.perY ? 0 : y);
+  const int ym = (y - 1 >= 0) ? y - 1 : (g.perY ? Ny - 1 : y);
+  const int zp = (z + 1 < Nz) ? z + 1 : (g.perZ ? 0 : z);
+  const int zm = (z - 1 >= 0) ? z - 1 : (g.perZ ? Nz - 1 : z);
+  const int sxy = Nx * Ny;
+  const int ixp = xp + y * Nx + z * sxy, ixm = xm + y * Nx + z * sxy;
+  const int iyp = x + yp * Nx + z * sxy, iym = x + ym * Nx + z * sxy;
+  const int izp = x + y * Nx + zp * sxy, izm = x + y * Nx + zm * sxy;
+  float src = 0.f;
+  if (flagged) {
+    const int2 r = sl.rng[i];
+    for (int k = r.x; k < r.x + r.y; ++k) {
+      const SpreadEntry e = sl.ent[k];
+      src += Sl[e.marker] * e.w;
+    }
+  }
+  const float ihx = 1.0f / p.dx, ihy = 1.0f / p.dy, ihz = 1.0f / p.dz;
+  const float ihx2 = ihx * ihx, ihy2 = ihy * ihy, ihz2 = ihz * ihz;
+  const float c = c1[i];
+  const float cxp = c1[ixp], cxm = c1[ixm], cyp = c1[iyp], cym = c1[iym], czp = c1[izp], czm = c1[izm];
+  const float3 u0 = u[i];
+  const float uL = 0.5f * (u[ixm].x + u0.x), uR = 0.5f * (u0.x + u[ixp].x);
+  const float vL = 0.5f * (u[iym].y + u0.y), vR = 0.5f * (u0.y + u[iyp].y);
+  const float wL = 0.5f * (u[izm].z + u0.z), wR = 0.5f * (u0.z + u[izp].z);
+  const float FLx = uL * ((uL >= 0.0f) ? cxm : c), FLy = vL * ((vL >= 0.0f) ? cym : c), FLz = wL * ((wL >= 0.0f) ? czm : c);
+  const float FRx = uR * ((uR >= 0.0f) ? c : cxp), FRy = vR * ((vR >= 0.0f) ? c : cyp), FRz = wR * ((wR >= 0.0f) ? c : czp);
+  const float conv = -((FRx - FLx) * ihx + (FRy - FLy) * ihy + (FRz - FLz) * ihz);
+  const float diff = p.D * ((cxm - 2.0f * c + cxp) * ihx2 + (cym - 2.0f * c + cyp) * ihy2 + (czm - 2.0f * c + czp) * ihz2);
+  c2[i] = c + p.dt * (conv + diff + src);
+}
+
+}  // namespace
+
+void launch_pack(const MarkerArrays& m, float g2l, hipStream_t s) {
+  if (m.M == 0) return;
+  k_pack<<<blocks_for(m.M, kThreads), kThreads, 0, s>>>(m, g2l);
+  GLSM_LAUNCH_CHECK();
+}
+void launch_scatter(const MarkerArrays& m, hipStream_t s) {
+  if (m.M == 0) return;
+  k_scatter<<<blocks_for(m.M, kThreads), kThreads, 0, s>>>(m);
+  GLSM_LAUNCH_CHECK();
+}
+void launch_repulsion(const MarkerArrays& m, float delta, float gamma, hipStream_t s) {
+  if (m.M == 0) return;
+  k_repulsion<<<blocks_for(m.M, kThreads), kThreads, 0, s>>>(m, delta, gamma);
+  GLSM_LAUNCH_CHECK();
+}
+void launch_ibm_interp(const LatticeGeom& g, const MarkerArrays& m, const float3* u, float beta_eff,
+                       hipStream_t s) {
+  if (m.M == 0) return;
+  k_ibm_interp<<<blocks_for(m.M, kWavesPerBlock), kThreads, 0, s>>>(g, m, u, beta_eff);
+  GLSM_LAUNCH_CHECK();
+}
+void launch_ibm_sample(const LatticeGeom& g, const MarkerArrays& m, const float* c, hipStream_t s) {
+  if (m.M == 0) return;
+  k_ibm_sample<<<blocks_for(m.M, kWavesPerBlock), kThreads, 0, s>>>(g, m, c);
+  GLSM_LAUNCH_CHECK();
+}
+
+void launch_spread_rebuild(const LatticeGeom& g, const MarkerArrays& m, const SpreadLists& sl, hipStream_t s) {
+  int* ctab = sl.ctab;
+  float* wtab = sl.wtab;
+  k_spread_clear_flags<<<blocks_for(sl.nblk, kThreads), kThreads, 0, s>>>(sl);
+  if (m.M > 0) k_spread_count<<<blocks_for(m.M, kWavesPerBlock), kThreads, 0, s>>>(g, m, sl, ctab, wtab);
+  k_spread_block_sum<<<sl.nblk, kThreads, 0, s>>>(g, sl);
+  k_spread_scan_blocks<<<1, kThreads, 0, s>>>(sl);
+  k_spread_local<<<sl.nblk, kThreads, 0, s>>>(g, sl);
+  if (m.M > 0) k_spread_fill<<<blocks_for((long)m.M * 64, kThreads), kThreads, 0, s>>>(m.M * 64, sl, ctab, wtab);
+  k_spread_sort<<<sl.nblk, kThreads, 0, s>>>(g, sl);
+  GLSM_LAUNCH_CHECK();
+}
+
+void launch_scalar(const LatticeGeom& g, const ScalarParams& p, const float3* u, const float* c1, float* c2,
+                   const SpreadLists& sl, const float* Sl, hipStream_t s) {
+  k_scalar<<<blocks_for(g.N, kThreads), kThreads, 0, s>>>(g, p, u, c1, c2, sl, Sl);
+  GLSM_LAUNCH_CHECK();
+}
+
+}  // namespace glsm

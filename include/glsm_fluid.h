@@ -125,6 +125,9 @@ int glsm_fluid_create(int size_x, int size_y, int size_z, int time, glsm_coupler
                       glsm_fluid** out);
 int glsm_fluid_destroy(glsm_fluid* f);                      /* fluid.cu:234-237 */
 int glsm_fluid_step_velocity(glsm_fluid* f, int iter);      /* fluid.cu:263-284 */
+/* one LBM sub-step (one pass of the fluid.cu:272-283 loop body) with the
+   given IBM stiffness; u is stored for every cell */
+int glsm_fluid_sub_step(glsm_fluid* f, float beta_eff);
 int glsm_fluid_step_concentration(glsm_fluid* f, int iter); /* fluid.cu:251-261 */
 int glsm_fluid_write_files(glsm_fluid* f, int iter);        /* fluid.cu:140-150 */
 /* FluidParams::perX/perY/perZ (fluid.h:21); honoured by stream, IBM and scalar */
@@ -132,6 +135,21 @@ int glsm_fluid_set_periodic(glsm_fluid* f, int per_x, int per_y, int per_z);
 /* FluidParams::beta and the IBM ramp length, normally from $IBM_BETA /
    $IBM_RAMP_STEPS (fluid.cu:198-201, 265-268) */
 int glsm_fluid_set_ibm(glsm_fluid* f, float beta, float ramp_steps);
+/* Extension: distribution storage of the post-collision state (collision is
+   FP32 in every mode).  The current state is converted.  scale_exp scales
+   FP16 deviations by 2^scale_exp (ignored otherwise). */
+enum glsm_storage {
+  GLSM_STORAGE_F32_FULL = 0, /* f, the reference's representation */
+  GLSM_STORAGE_F32_DEV = 1,  /* f - w_q in FP32 (default) */
+  GLSM_STORAGE_F16_DEV = 2,  /* (f - w_q) * 2^scale_exp in binary16 */
+  GLSM_STORAGE_BF16_DEV = 3  /* f - w_q in bfloat16 */
+};
+int glsm_fluid_set_storage(glsm_fluid* f, int storage, int scale_exp);
+/* Extension (measurement): sample the stream/collide launch duration with a
+   HIP event pair on the fluid's stream every `every`-th stepVelocity (0 =
+   off); the result call waits for the stream and returns the mean in ms. */
+int glsm_fluid_enable_lbm_timing(glsm_fluid* f, int every, int max_samples);
+int glsm_fluid_lbm_timing(glsm_fluid* f, double* mean_ms, int* samples);
 /* lattice (Nx,Ny,Nz) and Nsub (fluid.cu:167-197) */
 int glsm_fluid_dims(glsm_fluid* f, int grid[3], int* nsub);
 size_t glsm_fluid_field_bytes(glsm_fluid* f, int field);

@@ -1,0 +1,388 @@
+"""Pins of the CPU restatement (oracle/) against mathematics that fixes the
+answer independently of the code: lattice isotropy, the Peskin kernel's
+moment conditions, closed-form geometry of affinely deformed hexahedra,
+conservation laws of BGK + Guo forcing, textbook flows (Poiseuille, shear
+wave), IBM consistency (linear interpolation, spread normalisation), the
+discrete diffusion eigenvalue, a Lennard-Jones pair, and the published
+splitmix64 sequence.  The reference ships no tests or golden data, so these
+checks (and the fixtures the oracle writes) are what pins it.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+C = np.array([[0, 0, 0], [1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1],
+              [1, 1, 0], [-1, 1, 0], [-1, -1, 0], [1, -1, 0], [1, 0, 1], [-1, 0, 1], [-1, 0, -1],
+              [1, 0, -1], [0, 1, 1], [0, 1, -1], [0, -1, -1], [0, -1, 1]])
+
+
+# --------------------------------------------------------------- lattice
+def test_d3q19_tables_isotropy(oracle):
+    c, w, opp = oracle.lattice_tables()
+    assert len({tuple(v) for v in c}) == 19
+    assert sorted((c * c).sum(1).tolist()) == [0] + [1] * 6 + [2] * 12
+    assert abs(w.sum() - 1) < 1e-15
+    assert np.abs(w @ c).max() < 1e-15
+    second = np.einsum("q,qa,qb->ab", w, c, c)
+    assert np.abs(second - np.eye(3) / 3).max() < 1e-15
+    fourth = np.einsum("q,qa,qb,qc,qd->abcd", w, c, c, c, c)
+    d = np.eye(3)
+    iso = (np.einsum("ab,cd->abcd", d, d) + np.einsum("ac,bd->abcd", d, d) + np.einsum("ad,bc->abcd", d, d)) / 9
+    assert np.abs(fourth - iso).max() < 1e-15
+    assert np.array_equal(opp[opp], np.arange(19))
+    assert np.array_equal(c[opp], -c)
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_delta4_moment_conditions(oracle, prec):
+    tol = 1e-14 if prec == 64 else 3e-7
+    rng = np.random.default_rng(0)
+    for x in np.concatenate([rng.random(50), [0.0, 0.5, 0.25]]):
+        i = np.arange(-2, 4)
+        phi = np.array([oracle.delta4(x - k, prec) for k in i])
+        assert abs(phi.sum() - 1) < tol                 # partition of unity
+        assert abs(((x - i) * phi).sum()) < tol * 4     # first moment
+        assert abs((phi ** 2).sum() - 3 / 8) < tol * 4  # Peskin's sum of squares
+    assert oracle.delta4(0.0, prec) == 0.5
+    assert oracle.delta4(1.0, prec) == 0.25
+    assert oracle.delta4(2.0, prec) == 0.0 and oracle.delta4(2.5, prec) == 0.0
+
+
+# --------------------------------------------------------------- chemistry
+def fu_model(u, v, w, f=0.9, q=1e-4):
+    """Modified Oregonator activator rate of the gLSM model (Yashin & Balazs)."""
+    ww = (1 - w) ** 2
+    return ww * u - u * u - (1 - w) * f * v * (u - q * ww) / (u + q * ww)
+
+
+def fv_model(u, v, w):
+    return (1 - w) ** 2 * u - (1 - w) * v
+
+
+def test_rate_functions_match_model(oracle):
+    rng = np.random.default_rng(1)
+    for _ in range(100):
+        u, v, w = rng.uniform(0.01, 0.5), rng.uniform(0.01, 0.5), rng.uniform(0.05, 0.3)
+        assert abs(oracle.fu(u, v, w) - fu_model(u, v, w)) < 1e-15
+        assert abs(oracle.fv(u, v, w) - fv_model(u, v, w)) < 1e-15
+
+
+def test_steady_state_is_the_model_fixed_point(oracle):
+    from scipy.optimize import brentq
+
+    g = oracle.Gel((2, 2, 2), (5, 5, 5))
+    uss, vss, wss = g.steady_state()
+    assert wss == 0.139
+    assert abs(uss - vss / (1 - wss)) < 1e-15        # on the v-nullcline
+    assert abs(fv_model(uss, vss, wss)) < 1e-15
+    # independent root of the u-nullcline along u = v/(1-w), Brent's method
+    v_star = brentq(lambda v: fu_model(v / (1 - wss), v, wss), 1e-3, 0.9, xtol=1e-15)
+    assert abs(vss - v_star) < 1e-9
+    assert abs(uss - 0.075442021435) < 1e-11 and abs(vss - 0.064955580456) < 1e-11
+
+
+def test_chemistry_forward_euler_step(oracle):
+    g = oracle.Gel((4, 3, 2), (5, 5, 5))
+    nE = g.num_elements
+    L = g.node_grid
+    rng = np.random.default_rng(2)
+    um, vm = g.read("um"), g.read("vm")
+    interior = [x + y * (L[0] + 1) + z * (L[0] + 1) * (L[1] + 1)
+                for z in range(1, L[2]) for y in range(1, L[1]) for x in range(1, L[0])]
+    um[interior] = rng.uniform(0.05, 0.3, len(interior))
+    vm[interior] = rng.uniform(0.05, 0.3, len(interior))
+    g.write("um", um)
+    g.write("vm", vm)
+    g.step_chemistry(1)  # T0 = T1 = T2 = 0 before any elasticity step
+    w = 0.139
+    exp_u = um[interior] + 1e-3 * fu_model(um[interior], vm[interior], w)
+    exp_v = vm[interior] + 1e-3 * 0.3 * fv_model(um[interior], vm[interior], w)
+    assert np.abs(g.read("um")[interior] - exp_u).max() < 1e-15
+    assert np.abs(g.read("vm")[interior] - exp_v).max() < 1e-15
+    assert nE == (L[0] + 1) * (L[1] + 1) * (L[2] + 1)
+
+
+def test_chemistry_fixed_point_holds(oracle):
+    g = oracle.Gel((3, 3, 3), (5, 5, 5))
+    uss, vss, _ = g.steady_state()
+    for it in range(200):
+        g.step_chemistry(it)
+    L = g.node_grid
+    interior = [x + y * (L[0] + 1) + z * (L[0] + 1) * (L[1] + 1)
+                for z in range(1, L[2]) for y in range(1, L[1]) for x in range(1, L[0])]
+    assert np.abs(g.read("um")[interior] - uss).max() < 1e-12
+    assert np.abs(g.read("vm")[interior] - vss).max() < 1e-14
+
+
+# --------------------------------------------------------------- topology
+def surface_nodes(L):
+    out = []
+    for k in range(1, L[2] + 1):
+        for j in range(1, L[1] + 1):
+            for i in range(1, L[0] + 1):
+                if i in (1, L[0]) or j in (1, L[1]) or k in (1, L[2]):
+                    out.append(i + j * (L[0] + 2) + k * (L[0] + 2) * (L[1] + 2))
+    return np.array(out, dtype=np.int32)
+
+
+def type_histogram(e):
+    """Expected counts of the 27 type codes on a box with inner extents e."""
+    ex, ey, ez = e
+    h = np.zeros(27, dtype=int)
+    h[0] = ex * ey * ez
+    h[1] = h[2] = ey * ez
+    h[3] = h[4] = ex * ez
+    h[5] = h[6] = ex * ey
+    h[7:11] = ex
+    h[11:15] = ey
+    h[15:19] = ez
+    h[19:27] = 1
+    return h
+
+
+@pytest.mark.parametrize("size", [(1, 1, 1), (6, 6, 6), (8, 8, 8), (4, 4, 24), (3, 5, 2), (20, 20, 4)])
+def test_gel_topology(oracle, size):
+    g = oracle.Gel(size, (30, 30, 30))
+    L = tuple(s + 1 for s in size)
+    assert g.node_grid == L
+    assert g.num_nodes == np.prod([l + 2 for l in L]) and g.num_elements == np.prod([l + 1 for l in L])
+    assert g.boundary_count == L[0] * L[1] * L[2] - (L[0] - 2) * (L[1] - 2) * (L[2] - 2)
+    assert np.array_equal(g.read("bIndex"), surface_nodes(L))
+    assert np.array_equal(np.bincount(g.read("map_node"), minlength=27), type_histogram(L))
+    assert np.array_equal(np.bincount(g.read("map_element"), minlength=27), type_histogram([l - 1 for l in L]))
+
+
+# --------------------------------------------------------------- geometry / forces
+def test_element_geometry_under_affine_map(oracle):
+    g = oracle.Gel((3, 2, 4), (5, 6, 7))
+    rng = np.random.default_rng(3)
+    A = np.eye(3) + 0.1 * rng.standard_normal((3, 3))
+    b = rng.standard_normal(3)
+    rn = g.read("rn")
+    g.write("rn", rn @ A.T + b)
+    g.step_elasticity(0)  # ghost mirror keeps the map affine; geometry from these positions
+    detA = np.linalg.det(A)
+    assert np.abs(g.read("vol") - detA).max() < 1e-12
+    cof = detA * np.linalg.inv(A).T
+    normals = np.array([[-1, 0, 0], [1, 0, 0], [0, -1, 0], [0, 1, 0], [0, 0, -1], [0, 0, 1]], float)
+    nm = g.read("nmSm").reshape(-1, 6, 3)
+    assert np.abs(nm - (normals @ cof.T)[None]).max() < 1e-12
+    assert np.abs(g.read("wm") - 0.139 / detA).max() < 1e-12
+    # centroids: affine image of the undeformed element centres
+    L = g.node_grid
+    cz, cy, cx = np.meshgrid(np.arange(L[2] + 1), np.arange(L[1] + 1), np.arange(L[0] + 1), indexing="ij")
+    c0 = np.stack([5 + cx - L[0] / 2, 6 + cy - L[1] / 2, 7 + cz - L[2] / 2], -1).reshape(-1, 3)
+    assert np.abs(g.read("rm") - (c0 @ A.T + b)).max() < 1e-12
+
+
+def test_forces_vanish_inside_regular_lattice_and_are_mirror_symmetric(oracle):
+    g = oracle.Gel((4, 4, 4), (10, 10, 10))
+    g.step_elasticity(0)
+    L = g.node_grid
+    Fn = g.read("Fn").reshape(L[2] + 2, L[1] + 2, L[0] + 2, 3)
+    real = Fn[1:-1, 1:-1, 1:-1]
+    assert np.abs(real[1:-1, 1:-1, 1:-1]).max() < 1e-15
+    assert np.abs(real).max() > 1e-6  # the surface feels the osmotic pressure
+    # reflection x -> -x flips the x component only
+    assert np.abs(real[:, :, ::-1, 0] + real[:, :, :, 0]).max() < 1e-15
+    assert np.abs(real[:, :, ::-1, 1] - real[:, :, :, 1]).max() < 1e-15
+
+
+# --------------------------------------------------------------- lattice Boltzmann
+def feq(rho, u):
+    cu = u @ C.T
+    uu = (u * u).sum(1)[:, None]
+    w = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12)
+    return (w[None] * rho[:, None] * (1 + 3 * cu + 4.5 * cu * cu - 1.5 * uu)).T
+
+
+def empty_fluid(oracle, size, prec=64, per=(0, 0, 0)):
+    cp = oracle.Coupler([], prec)
+    fl = oracle.Fluid(size, cp)
+    fl.set_periodic(*per)
+    return cp, fl
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_rest_state_is_fixed_point(oracle, prec):
+    cp, fl = empty_fluid(oracle, (3, 2, 4), prec)
+    f0 = fl.read("f")
+    for _ in range(10):
+        fl.sub_step(1.0)
+    # fixed up to the rounding of sum(w_q) = 1 in the working precision
+    eps = np.finfo(np.float64 if prec == 64 else np.float32).eps
+    assert np.abs(fl.read("f") - f0).max() <= 4 * eps
+    assert np.abs(fl.read("u")).max() <= 4 * eps
+
+
+def test_mass_conserved_with_walls(oracle):
+    cp, fl = empty_fluid(oracle, (4, 3, 5))
+    rng = np.random.default_rng(4)
+    f0 = feq(1 + 0.01 * rng.standard_normal(fl.N), 0.02 * rng.standard_normal((fl.N, 3)))
+    fl.write("f", f0)
+    m0 = f0.sum()
+    for _ in range(20):
+        fl.sub_step(1.0)
+    assert abs(fl.read("f").sum() - m0) / m0 < 1e-13
+
+
+def test_guo_forcing_adds_exact_momentum(oracle):
+    cp, fl = empty_fluid(oracle, (4, 3, 5), per=(1, 1, 1))
+    rng = np.random.default_rng(5)
+    f0 = feq(1 + 0.01 * rng.standard_normal(fl.N), 0.02 * rng.standard_normal((fl.N, 3)))
+    fl.write("f", f0)
+    F = np.array([1e-5, -2e-5, 3e-5])
+    fl.set_force(*F)
+    p0 = (f0.T @ C).sum(0)
+    fl.sub_step(1.0)
+    p1 = (fl.read("f").T @ C).sum(0)
+    assert np.abs((p1 - p0) - fl.N * F).max() < 1e-12 * fl.N
+
+
+def test_poiseuille_profile(oracle):
+    """Body-force channel between halfway bounce-back walls at y=-1/2 and
+    y=Ny-1/2: u_x(y) = F/(2 nu) (y+1/2)(Ny-1/2-y), nu = (tau-1/2)/3 = 0.1."""
+    cp, fl = empty_fluid(oracle, (1, 8, 1), per=(1, 0, 1))
+    Nx, Ny, Nz = fl.grid
+    F = 1e-6
+    fl.set_force(F, 0, 0)
+    for _ in range(6000):
+        fl.sub_step(0.0)
+    ux = fl.read("u")[:, 0].reshape(Nz, Ny, Nx)[1, :, 1]
+    y = np.arange(Ny)
+    exact = F / (2 * 0.1) * (y + 0.5) * (Ny - 0.5 - y)
+    # BGK halfway bounce-back carries an O(1e-3) slip at tau = 0.8
+    assert rel_l2(ux, exact) < 5e-3
+    assert abs(ux.sum() / exact.sum() - 1) < 5e-3
+
+
+def test_shear_wave_decays_at_viscous_rate(oracle):
+    cp, fl = empty_fluid(oracle, (1, 16, 1), per=(1, 1, 1))
+    Nx, Ny, Nz = fl.grid
+    y = np.tile(np.repeat(np.arange(Ny), Nx), Nz)
+    k = 2 * np.pi / Ny
+    A = 1e-3
+    u = np.zeros((fl.N, 3))
+    u[:, 0] = A * np.sin(k * y)
+    fl.write("f", feq(np.ones(fl.N), u))
+    T = 300
+    for _ in range(T):
+        fl.sub_step(0.0)
+    ux = fl.read("u")[:, 0]
+    amp = 2 * (ux * np.sin(k * y)).sum() / fl.N
+    assert abs(np.log(amp / A) / (-0.1 * k * k * T) - 1) < 2e-3
+
+
+# --------------------------------------------------------------- IBM
+def gel_in_box(oracle, prec=64, per=(0, 0, 0), size=(3, 3, 3), pos=(8, 8, 8), box=(16, 16, 16)):
+    g = oracle.Gel(size, pos)
+    cp = oracle.Coupler([g], prec)
+    fl = oracle.Fluid(box, cp)
+    fl.set_periodic(*per)
+    g.step_elasticity(0)
+    cp.pack_from_gels()
+    return g, cp, fl
+
+
+def test_interpolation_reproduces_linear_field(oracle):
+    g, cp, fl = gel_in_box(oracle)
+    Nx, Ny, Nz = fl.grid
+    z, y, x = np.meshgrid(np.arange(Nz), np.arange(Ny), np.arange(Nx), indexing="ij")
+    X = np.stack([x, y, z], -1).reshape(-1, 3).astype(float)
+    a = np.array([1e-3, -2e-3, 5e-4])
+    B = 1e-4 * np.array([[1, 2, -1], [0.5, -1, 2], [3, 0, 1]])
+    fl.write("u", a + X @ B.T)
+    fl.sub_step(0.0)
+    lag = cp.read("lag")
+    assert np.abs(cp.read("Ul") - (a + (lag / 0.5) @ B.T)).max() < 1e-15
+
+
+def test_spread_adds_eight_times_marker_force(oracle):
+    """sum_x F_ibm = (1/h^3) sum_l F_l, seen as the momentum gain of one
+    sub-step in a periodic box (Guo: sum c f' = sum c f + sum F)."""
+    g, cp, fl = gel_in_box(oracle, per=(1, 1, 1))
+    rng = np.random.default_rng(6)
+    f0 = feq(np.ones(fl.N), 1e-3 * rng.standard_normal((fl.N, 3)))
+    fl.write("f", f0)
+    fl.write("u", 1e-3 * rng.standard_normal((fl.N, 3)))
+    p0 = (f0.T @ C).sum(0)
+    fl.sub_step(0.7)
+    dp = (fl.read("f").T @ C).sum(0) - p0
+    Fl = cp.read("Fdrag")
+    assert np.abs(Fl).max() > 1e-6
+    assert np.abs(dp - 8 * Fl.sum(0)).max() < 1e-13
+
+
+# --------------------------------------------------------------- scalar
+def test_scalar_diffusion_eigenmode(oracle):
+    cp, fl = empty_fluid(oracle, (8, 1, 1), per=(1, 1, 1))
+    Nx, Ny, Nz = fl.grid
+    x = np.tile(np.arange(Nx), Ny * Nz)
+    k = 2 * np.pi / Nx
+    c0 = np.cos(k * x)
+    fl.write("c", c0)
+    fl.step_concentration(0)
+    lam = 1 + 1e-3 * (2 * np.cos(k) - 2)
+    assert np.abs(fl.read("c") - lam * c0).max() < 1e-15
+
+
+def test_scalar_source_is_spread_and_conserved(oracle):
+    g, cp, fl = gel_in_box(oracle, per=(1, 1, 1))
+    fl.write("c", np.full(fl.N, 0.25))
+    c0 = fl.read("c").sum()
+    fl.step_concentration(0)
+    Sl = cp.read("Sl")
+    assert np.abs(Sl).max() > 1e-3
+    assert abs(fl.read("c").sum() - (c0 + 1e-3 * 8 * Sl.sum())) < 1e-10
+
+
+def test_constant_scalar_invariant_with_walls(oracle):
+    cp, fl = empty_fluid(oracle, (3, 4, 2))
+    fl.write("c", np.full(fl.N, 0.3))
+    fl.step_concentration(0)
+    assert np.abs(fl.read("c") - 0.3).max() < 1e-16
+
+
+# --------------------------------------------------------------- coupler
+@pytest.mark.parametrize("prec", [64, 32])
+def test_lennard_jones_pair(oracle, prec):
+    ga, gb = oracle.Gel((1, 1, 1), (0, 0, 0)), oracle.Gel((1, 1, 1), (0, 0, 0))
+    cp = oracle.Coupler([ga, gb], prec)
+    lag = np.array([[100.0 + 10 * l, 50.0, 50.0] for l in range(16)])
+    lag[0] = [0.0, 0.0, 0.0]
+    lag[8] = [2.0, 0.5, -0.25]
+    cp.write("lag", lag)
+    cp.apply_gel_repulsion()
+    Fr = cp.read("Frep")
+    d = lag[0] - lag[8]
+    r2 = d @ d
+    sig2 = 1.5 ** 2
+    s6 = (sig2 / r2) ** 3
+    delta = float(np.float32(1e-4))  # CouplerParams::delta is a float (coupling.cu:146)
+    fac = 24 * delta * (2 * s6 * s6 - s6) / r2
+    tol = 1e-13 if prec == 64 else 1e-6
+    assert np.abs(Fr[0] - fac * d).max() < tol * abs(fac)
+    assert np.abs(Fr[8] + fac * d).max() < tol * abs(fac)
+    assert np.abs(np.delete(Fr, [0, 8], axis=0)).max() == 0.0
+
+
+# --------------------------------------------------------------- seeded perturbation
+def test_perturbation_uses_splitmix64(oracle):
+    g = oracle.Gel((2, 2, 2), (5, 5, 5))
+    uss, vss, _ = g.steady_state()
+    um0, vm0 = g.read("um"), g.read("vm")
+    g.perturb_chemistry(0, 0.01)
+    L = g.node_grid
+    first = 1 + 1 * (L[0] + 1) + 1 * (L[0] + 1) * (L[1] + 1)
+    second = 2 + 1 * (L[0] + 1) + 1 * (L[0] + 1) * (L[1] + 1)
+    # published splitmix64 outputs for seed 0
+    z = [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F, 0xF88BB8A8724C81EC]
+    unit = [(v >> 11) * 2.0 ** -53 for v in z]
+    assert g.read("um")[first] == um0[first] + 0.01 * (2 * unit[0] - 1)
+    assert g.read("vm")[first] == vm0[first] + 0.01 * (2 * unit[1] - 1)
+    assert g.read("um")[second] == um0[second] + 0.01 * (2 * unit[2] - 1)
+    assert g.read("vm")[second] == vm0[second] + 0.01 * (2 * unit[3] - 1)
+    assert abs(um0[first] - uss) == 0 and abs(vm0[first] - vss) == 0
