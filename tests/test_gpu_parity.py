@@ -49,11 +49,23 @@ GEL_FIELDS_CMP = ["rn", "um", "vm", "wm", "um_norm", "vm_norm", "wmp", "T0", "T1
 
 
 def assert_gel_close(g, og, tol=1e-12, fields=GEL_FIELDS_CMP):
+    """Each field within `tol` of the oracle, relative to the field's largest
+    magnitude.  T1 = (sum of the six face area vectors) . vmum / vol vanishes
+    identically for a closed hexahedron (the 2x2 rule is exact for bilinear
+    faces), so both sides hold rounding noise; it is compared on the scale of
+    its terms, |nmSm| |Veln| |un_norm| / vol."""
+    bad = []
     for name in fields:
         a, b = g.read(name), og.read(name)
-        scale = max(np.abs(b).max(), 1e-300)
-        err = np.abs(a - b).max() / scale
-        assert err <= tol, f"{name}: max rel err {err:.3e}"
+        if name == "T1":
+            scale = 8 * np.abs(og.read("nmSm")).max() * np.abs(og.read("Veln")).max() * \
+                np.abs(og.read("un_norm")).max() / np.abs(og.read("vol")).max()
+        else:
+            scale = np.abs(b).max()
+        err = np.abs(a - b).max() / max(scale, 1e-300)
+        if err > tol:
+            bad.append(f"{name}: {err:.3e}")
+    assert not bad, "max relative errors: " + ", ".join(bad)
 
 
 # ------------------------------------------------------------------ topology
@@ -250,8 +262,10 @@ def test_config0_200_steps_vs_fp64(glsm, oracle):
     assert errs["bf16"] <= 1e-2
     assert rel_l2(f.read("c"), co) <= 1e-6
     assert np.abs(g.read("rn") - og.read("rn")).max() <= 5e-7
-    for name in ("um", "vm", "wm", "un_norm"):
+    for name in ("um", "vm", "wm"):
         assert rel_l2(g.read(name), og.read(name)) <= 1e-9, name
+    # un_norm carries un_robin = -Sl from the FP32 concentration exchange
+    assert rel_l2(g.read("un_norm"), og.read("un_norm")) <= 1e-6
 
 
 def test_config0_1000_steps_vs_golden(glsm):
@@ -268,8 +282,9 @@ def test_config0_1000_steps_vs_golden(glsm):
     assert eu <= 1e-5
     assert rel_l2(f.read("c"), gold["c_1000"]) <= 1e-6
     assert np.abs(g.read("rn") - gold["rn_1000"]).max() <= 5e-7
-    for name in ("um", "vm", "wm", "un_norm"):
+    for name in ("um", "vm", "wm"):
         assert rel_l2(g.read(name), gold[f"{name}_1000"]) <= 1e-9, name
+    assert rel_l2(g.read("un_norm"), gold["un_norm_1000"]) <= 1e-6
 
 
 def test_walled_two_gels_vs_fp64(glsm, oracle):

@@ -178,6 +178,24 @@ def test_element_geometry_under_affine_map(oracle):
     assert np.abs(g.read("rm") - (c0 @ A.T + b)).max() < 1e-12
 
 
+def test_face_area_vectors_close_so_T1_vanishes(oracle):
+    """The six face area vectors of any (deformed) hexahedron sum to zero --
+    a closed surface -- and the 2x2 rule is exact for bilinear faces; T1 =
+    (sum of faces) . vmum / vol is therefore zero up to rounding."""
+    g = oracle.Gel((5, 4, 3), (10, 10, 10))
+    rng = np.random.default_rng(9)
+    rn = g.read("rn")
+    g.write("rn", rn + 0.05 * rng.standard_normal(rn.shape))
+    for it in range(6):
+        g.step_elasticity(it)
+        g.step_chemistry(it)
+    nm = g.read("nmSm").reshape(-1, 6, 3)
+    assert np.abs(nm).max() > 0.5
+    assert np.abs(nm.sum(1)).max() < 1e-14
+    assert np.abs(g.read("Veln")).max() > 1e-3
+    assert np.abs(g.read("T1")).max() < 1e-14
+
+
 def test_forces_vanish_inside_regular_lattice_and_are_mirror_symmetric(oracle):
     g = oracle.Gel((4, 4, 4), (10, 10, 10))
     g.step_elasticity(0)
