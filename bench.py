@@ -139,7 +139,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--extra-steps", type=int, default=400)
+    ap.add_argument("--workload", default="all", choices=["all", "101", "201-f32", "201-f16", "201-bf16"],
+                    help="profile one workload alone (no CPU baseline)")
     args = ap.parse_args()
+    if args.workload != "all":
+        args.no_cpu_baseline = True
+        args.no_extra = args.workload == "101"
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the time step runs on a single "
               "device and multi-GPU runs are not part of this build.", file=sys.stderr)
@@ -148,6 +153,15 @@ def main():
     import glsm_fluid_3d_amd as G
 
     G.init(0)
+    if args.workload.startswith("201"):
+        st = args.workload.split("-")[1]
+        gels, c, f = config2b(G, st)
+        dt2, ms2, n2 = run_gpu(G, gels, c, f, args.steps, args.warmup)
+        v2 = f.N * f.nsub * args.steps / dt2 / 1e6
+        print(json.dumps({"workload": f"201^3, four 8^3 gels, {st}", "value": round(v2, 1), "unit": "MLUPS",
+                          "steps": args.steps, "ms_per_step": round(dt2 / args.steps * 1e3, 4),
+                          "roofline": roofline(f.N, 2 * 19 * G.STORAGE_BYTES[st], ms2)}), flush=True)
+        return
     gels, c, f = config1(G, storage=args.storage)
     N = f.N
     log(f"workload 101^3: N={N} nsub={f.nsub} markers={c.M} storage={args.storage}")

@@ -150,6 +150,8 @@ Fluid::Fluid(int3 size, int time, Coupler* cp)
   d_ent_ = glsm::dalloc<SpreadEntry>(ent_cap_);
   d_ctab_ = glsm::dalloc<int>(ent_cap_);
   d_wtab_ = glsm::dalloc<float>(ent_cap_);
+  d_rtab_ = glsm::dalloc<float>(ent_cap_);
+  d_wsum_ = glsm::dalloc<float>(std::max(1, coupler->sumGelBoundaryCount));
 
   glsm::launch_lbm_init(storage_, geom(*this), lbm_params(*h_fp, storage_, scale_exp_), d_f, plane, d_u, d_c1, d_c2,
                         fluid_stream);
@@ -165,7 +167,8 @@ Fluid::~Fluid() {
   (void)hipHostFree(h_c1);
   delete h_fp;
   for (void* p : {(void*)d_f, (void*)d_fnext, (void*)d_u, (void*)d_c1, (void*)d_c2, (void*)d_blkflag_, (void*)d_cnt_,
-                  (void*)d_fillpos_, (void*)d_rng_, (void*)d_bsum_, (void*)d_ent_, (void*)d_ctab_, (void*)d_wtab_})
+                  (void*)d_fillpos_, (void*)d_rng_, (void*)d_bsum_, (void*)d_ent_, (void*)d_ctab_, (void*)d_wtab_,
+                  (void*)d_rtab_, (void*)d_wsum_})
     glsm::dfree(p);
 }
 
@@ -215,6 +218,8 @@ glsm::SpreadLists Fluid::spreadLists() const {
   sl.ent = d_ent_;
   sl.ctab = d_ctab_;
   sl.wtab = d_wtab_;
+  sl.rtab = d_rtab_;
+  sl.wsum = d_wsum_;
   return sl;
 }
 
@@ -236,7 +241,7 @@ void Fluid::stepVelocity(int iter) {
   const bool timed = timing_every_ > 0 && (timing_calls_++ % timing_every_) == 0 &&
                      (int)timing_events_.size() < 2 * timing_max_;
   for (int kk = 0; kk < Nsub; kk++) {
-    glsm::launch_ibm_interp(g, m, d_u, beta_eff, fluid_stream);
+    glsm::launch_ibm_interp(m, sl, d_u, beta_eff, fluid_stream);
     hipEvent_t ev[2] = {nullptr, nullptr};
     if (timed && kk == Nsub / 2) {
       for (auto& e : ev) GLSM_HIP(hipEventCreate(&e));
@@ -277,7 +282,7 @@ double Fluid::lbmKernelTiming(int* samples) {
 void Fluid::subStep(float beta_eff, bool write_u_all) {
   ensureSpreadLists();
   const glsm::MarkerArrays m = glsm::markers_of(*coupler);
-  glsm::launch_ibm_interp(geom(*this), m, d_u, beta_eff, fluid_stream);
+  glsm::launch_ibm_interp(m, spreadLists(), d_u, beta_eff, fluid_stream);
   glsm::launch_lbm_step(storage_, geom(*this), lbm_params(*h_fp, storage_, scale_exp_), d_f, d_fnext, plane, d_u,
                         write_u_all, spreadLists(), m.Fdrag, fluid_stream);
   std::swap(d_f, d_fnext);
@@ -303,7 +308,7 @@ void Fluid::stepConcentration(int /*iter*/) {
   ensureSpreadLists();
   const glsm::LatticeGeom g = geom(*this);
   const glsm::MarkerArrays m = glsm::markers_of(*coupler);
-  glsm::launch_ibm_sample(g, m, d_c1, fluid_stream);
+  glsm::launch_ibm_sample(m, spreadLists(), d_c1, fluid_stream);
   glsm::ScalarParams sp{h_fp->dt, h_fp->D, h_fp->dx, h_fp->dy, h_fp->dz};
   glsm::launch_scalar(g, sp, d_u, d_c1, d_c2, spreadLists(), m.Sl, fluid_stream);
   std::swap(d_c1, d_c2);

@@ -120,18 +120,21 @@ struct SpreadLists {
   int2* rng;
   int* bsum;  // nblk + 1
   SpreadEntry* ent;
-  int* ctab;    // 64 per marker: stencil cell or -1 (rebuild scratch)
-  float* wtab;  // 64 per marker: spreading weight (rebuild scratch)
+  // per marker, rebuilt with the lists (markers are fixed between rebuilds):
+  int* ctab;     // 64 per marker: stencil cell (x outer, z inner) or -1
+  float* wtab;   // 64 per marker: spreading weight phi * (1/h^3) / sum(phi)
+  float* rtab;   // 64 per marker: interpolation weight phi_x phi_y phi_z
+  float* wsum;   // per marker: sum of rtab in stencil order
 };
 
 // rebuild the per-cell contribution lists for the current marker positions
 void launch_spread_rebuild(const LatticeGeom& g, const MarkerArrays& m, const SpreadLists& sl,
                            hipStream_t s);
-// Ul = interp(u), Fdrag = beta_eff (Vl - Ul)
-void launch_ibm_interp(const LatticeGeom& g, const MarkerArrays& m, const float3* u, float beta_eff,
+// Ul = interp(u), Fdrag = beta_eff (Vl - Ul), from the rebuilt stencil tables
+void launch_ibm_interp(const MarkerArrays& m, const SpreadLists& sl, const float3* u, float beta_eff,
                        hipStream_t s);
 // Dl = interp(c), Sl = Cl - Dl
-void launch_ibm_sample(const LatticeGeom& g, const MarkerArrays& m, const float* c, hipStream_t s);
+void launch_ibm_sample(const MarkerArrays& m, const SpreadLists& sl, const float* c, hipStream_t s);
 
 // distribution storage representations (collision is FP32 in all of them)
 enum StorageMode { F32_FULL = 0, F32_DEV = 1, F16_DEV = 2, BF16_DEV = 3 };

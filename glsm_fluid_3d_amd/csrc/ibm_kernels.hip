@@ -7,13 +7,18 @@
 //
 // Design (vs. the reference's one-thread-per-marker loops with atomicAdd,
 // fluid_kernels.cu:244-459):
-//   * interpolation: one 64-lane wavefront per marker, one lane per cell of
-//     the 4x4x4 Peskin stencil; the 64 products are summed in stencil order
-//     by lane 0 from LDS, which keeps the reference's summation order;
-//   * spreading: gathered, not scattered.  When the markers move, per-cell
-//     lists of (marker, weight) are rebuilt (count, block scan, fill, sort by
-//     marker).  The fused LBM kernel and the scalar kernel then sum a cell's
-//     contributions in marker order -- no atomics, the same bits every run.
+//   * markers only move between list rebuilds, so each marker's 64 stencil
+//     cells (4x4x4 Peskin support, wall-clipped or wrapped), its raw weights
+//     phi_x phi_y phi_z, their sum and the spreading weights are tabulated
+//     once per rebuild;
+//   * interpolation: one 64-lane wavefront per marker, one lane per stencil
+//     cell; the 64 products are summed in stencil order with lane reads, which
+//     keeps the reference's summation order without LDS round trips;
+//   * spreading: gathered, not scattered.  Per-cell lists of (marker,
+//     weight), sorted by marker, are rebuilt with the tables (count, block
+//     scan, fill, sort).  The fused LBM kernel and the scalar kernel sum a
+//     cell's contributions in marker order -- no atomics, the same bits every
+//     run.
 #include "glsm_internal.hpp"
 
 namespace glsm {
@@ -24,6 +29,10 @@ constexpr int kWavesPerBlock = kThreads / 64;
 constexpr int kFlagCells = 256;  // cells per spreading block flag
 
 inline int blocks_for(long n, int per) { return (int)((n + per - 1) / per); }
+
+__device__ inline float lane_read(float v, int k) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
+}
 
 // Peskin 4-point kernel, fluid_kernels.cu:60-71
 __device__ inline float delta4(float r) {
@@ -67,6 +76,30 @@ __device__ inline StencilLane stencil_lane(const LatticeGeom& g, float3 p, int l
   return s;
 }
 
+// exclusive scan of one int per thread over the 256-thread block:
+// wavefront shuffles, then one LDS exchange of the four wave totals
+__device__ inline int block_excl_scan(int v, int* s_wave, int& total) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) s_wave[w] = x;
+  __syncthreads();
+  int pre = 0;
+  total = 0;
+#pragma unroll
+  for (int k = 0; k < kWavesPerBlock; ++k) {
+    const int t = s_wave[k];
+    if (k < w) pre += t;
+    total += t;
+  }
+  __syncthreads();
+  return pre + x - v;
+}
+
 // ---------------------------------------------------------------- coupler
 // k_gather_boundary, coupling_kernels.cu:42-59, all gels in one launch
 __global__ void __launch_bounds__(kThreads) k_pack(MarkerArrays m, float g2l) {
@@ -100,77 +133,77 @@ __global__ void __launch_bounds__(kThreads) k_scatter(MarkerArrays m) {
 }
 
 // Lennard-Jones repulsion between markers of different gels
-// (k_gel_repulsion / bodyBodyInteraction, coupling_kernels.cu:61-146):
-// positions staged through LDS tiles, partners visited in index order, the
-// result written once per marker (no atomics).
+// (k_gel_repulsion / bodyBodyInteraction, coupling_kernels.cu:61-146).  One
+// wavefront per marker a; its 64 lanes evaluate 64 partners at a time, and
+// the partners inside the cut-off (a ballot mask) are added in increasing
+// index -- the reference's accumulation order -- by every lane alike.
 __global__ void __launch_bounds__(kThreads) k_repulsion(MarkerArrays m, float delta, float gamma) {
-  __shared__ float4 s_pos[kThreads];
-  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int a = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  if (a >= m.M) return;  // whole wavefront
   const float rcut = 2.5f * gamma, rc2 = rcut * rcut, sig2 = gamma * gamma;
-  float3 pa = make_float3(0.f, 0.f, 0.f);
-  int oa = -1;
-  if (a < m.M) {
-    pa = m.lag[a];
-    oa = m.owner[a];
-  }
+  const float3 pa = m.lag[a];
+  const int oa = m.owner[a];
   float fx = 0.f, fy = 0.f, fz = 0.f;
-  for (int base = 0; base < m.M; base += kThreads) {
-    const int b = base + threadIdx.x;
-    if (b < m.M) {
+  for (int base = 0; base < m.M; base += 64) {
+    const int b = base + lane;
+    float cx = 0.f, cy = 0.f, cz = 0.f;
+    bool hit = false;
+    if (b < m.M && m.owner[b] != oa) {
       const float3 pb = m.lag[b];
-      s_pos[threadIdx.x] = make_float4(pb.x, pb.y, pb.z, __int_as_float(m.owner[b]));
-    } else {
-      s_pos[threadIdx.x] = make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
-    }
-    __syncthreads();
-    if (a < m.M) {
-      const int nb = min(kThreads, m.M - base);
-      for (int k = 0; k < nb; ++k) {
-        const float4 q = s_pos[k];
-        if (__float_as_int(q.w) == oa) continue;
-        const float dx = pa.x - q.x, dy = pa.y - q.y, dz = pa.z - q.z;
-        const float d2 = dx * dx + dy * dy + dz * dz;
-        if (d2 <= 0.0f || d2 >= rc2) continue;
+      const float dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
+      const float d2 = dx * dx + dy * dy + dz * dz;
+      if (d2 > 0.0f && d2 < rc2) {
         const float inv = 1.0f / d2;
         const float sr2 = sig2 * inv;
         const float sr6 = sr2 * sr2 * sr2;
         const float sr12 = sr6 * sr6;
         const float fac = 24.0f * delta * (2.0f * sr12 - sr6) * inv;
-        fx += fac * dx;
-        fy += fac * dy;
-        fz += fac * dz;
+        cx = fac * dx;
+        cy = fac * dy;
+        cz = fac * dz;
+        hit = true;
       }
     }
-    __syncthreads();
+    unsigned long long mask = __ballot(hit);
+    while (mask) {
+      const int k = __ffsll(mask) - 1;
+      fx += lane_read(cx, k);
+      fy += lane_read(cy, k);
+      fz += lane_read(cz, k);
+      mask &= mask - 1;
+    }
   }
-  if (a < m.M) m.Frep[a] = make_float3(0.f + fx, 0.f + fy, 0.f + fz);
+  if (lane == 0) m.Frep[a] = make_float3(0.f + fx, 0.f + fy, 0.f + fz);
 }
 
 // ---------------------------------------------------------------- IBM
 // Ul = sum(u*phi)/sum(phi) over the stencil, Fdrag = beta_eff*(Vl - Ul)
 // (k_ibm_interpolate_velocity + k_scale_negbeta, fluid_kernels.cu:244-291)
-__global__ void __launch_bounds__(kThreads) k_ibm_interp(LatticeGeom g, MarkerArrays m, const float3* __restrict__ u,
+__global__ void __launch_bounds__(kThreads) k_ibm_interp(MarkerArrays m, SpreadLists sl, const float3* __restrict__ u,
                                                          float beta_eff) {
-  __shared__ float4 s[kWavesPerBlock][64];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int l = blockIdx.x * kWavesPerBlock + wv;
-  const bool live = l < m.M;
-  float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-  if (live) {
-    const StencilLane st = stencil_lane(g, m.lag[l], lane);
-    if (st.cell >= 0) {
-      const float3 uc = u[st.cell];
-      v = make_float4(uc.x * st.w, uc.y * st.w, uc.z * st.w, st.w);
-    }
+  const int lane = threadIdx.x & 63;
+  const int l = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  if (l >= m.M) return;  // whole wavefront
+  const size_t k = (size_t)l * 64 + lane;
+  const int cell = sl.ctab[k];
+  float px = 0.f, py = 0.f, pz = 0.f;
+  if (cell >= 0) {
+    const float w = sl.rtab[k];
+    const float3 uc = u[cell];
+    px = uc.x * w;
+    py = uc.y * w;
+    pz = uc.z * w;
   }
-  s[wv][lane] = v;
-  __syncthreads();
-  if (live && lane == 0) {
-    float ux = 0.f, uy = 0.f, uz = 0.f, ws = 0.f;
-    for (int k = 0; k < 64; ++k) {
-      const float4 t = s[wv][k];
-      ux += t.x; uy += t.y; uz += t.z; ws += t.w;
-    }
+  float ux = 0.f, uy = 0.f, uz = 0.f;
+#pragma unroll
+  for (int j = 0; j < 64; ++j) {
+    ux += lane_read(px, j);
+    uy += lane_read(py, j);
+    uz += lane_read(pz, j);
+  }
+  if (lane == 0) {
+    const float ws = sl.wsum[l];
     float3 ul = make_float3(0.f, 0.f, 0.f);
     if (ws > 0.0f) ul = make_float3(ux / ws, uy / ws, uz / ws);
     m.Ul[l] = ul;
@@ -181,24 +214,18 @@ __global__ void __launch_bounds__(kThreads) k_ibm_interp(LatticeGeom g, MarkerAr
 
 // Dl = sum(c*phi)/sum(phi), Sl = Cl - Dl (k_ibm_sample_concentration +
 // k_robin_boundary, fluid_kernels.cu:350-387, 293-298)
-__global__ void __launch_bounds__(kThreads) k_ibm_sample(LatticeGeom g, MarkerArrays m, const float* __restrict__ c) {
-  __shared__ float2 s[kWavesPerBlock][64];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int l = blockIdx.x * kWavesPerBlock + wv;
-  const bool live = l < m.M;
-  float2 v = make_float2(0.f, 0.f);
-  if (live) {
-    const StencilLane st = stencil_lane(g, m.lag[l], lane);
-    if (st.cell >= 0) v = make_float2(c[st.cell] * st.w, st.w);
-  }
-  s[wv][lane] = v;
-  __syncthreads();
-  if (live && lane == 0) {
-    float cl = 0.f, ws = 0.f;
-    for (int k = 0; k < 64; ++k) {
-      cl += s[wv][k].x;
-      ws += s[wv][k].y;
-    }
+__global__ void __launch_bounds__(kThreads) k_ibm_sample(MarkerArrays m, SpreadLists sl, const float* __restrict__ c) {
+  const int lane = threadIdx.x & 63;
+  const int l = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  if (l >= m.M) return;
+  const size_t k = (size_t)l * 64 + lane;
+  const int cell = sl.ctab[k];
+  const float p = cell >= 0 ? c[cell] * sl.rtab[k] : 0.f;
+  float cl = 0.f;
+#pragma unroll
+  for (int j = 0; j < 64; ++j) cl += lane_read(p, j);
+  if (lane == 0) {
+    const float ws = sl.wsum[l];
     const float dl = ws > 0.0f ? cl / ws : 0.0f;
     m.Dl[l] = dl;
     m.Sl[l] = 1 * (m.Cl[l] - dl);
@@ -206,38 +233,31 @@ __global__ void __launch_bounds__(kThreads) k_ibm_sample(LatticeGeom g, MarkerAr
 }
 
 // ---- spreading lists ----
-// Per marker: the spreading weight of each stencil cell, phi*(1/h^3)/sum(phi)
-// (k_ibm_spread_forces, fluid_kernels.cu:300-348); a marker whose weights
-// sum to zero spreads nothing.  Counts contributions per cell and flags the
+// Per marker: stencil cells and weights; the weight sum in stencil order;
+// the spreading weight phi*(1/h^3)/sum(phi) (k_ibm_spread_forces,
+// fluid_kernels.cu:300-348); a marker whose weights sum to zero neither
+// interpolates nor spreads.  Counts contributions per cell and flags the
 // 256-cell blocks that receive any.
-__global__ void __launch_bounds__(kThreads) k_spread_count(LatticeGeom g, MarkerArrays m, SpreadLists sl,
-                                                           int* __restrict__ ctab, float* __restrict__ wtab) {
-  __shared__ float s[kWavesPerBlock][64];
-  __shared__ float s_scale[kWavesPerBlock];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int l = blockIdx.x * kWavesPerBlock + wv;
-  const bool live = l < m.M;
-  StencilLane st{-1, 0.f};
-  if (live) st = stencil_lane(g, m.lag[l], lane);
-  s[wv][lane] = st.w;
-  __syncthreads();
-  if (lane == 0) {
-    float ws = 0.f;
-    for (int k = 0; k < 64; ++k) ws += s[wv][k];
-    const float inv_h3 = (float)(1.0 / (double)(g.h * g.h * g.h));
-    s_scale[wv] = ws == 0.0f ? 0.0f : 1 * inv_h3 / ws;
-    if (ws == 0.0f) s_scale[wv] = -1.0f;  // marker spreads nothing
-  }
-  __syncthreads();
-  if (!live) return;
-  const float scale = s_scale[wv];
+__global__ void __launch_bounds__(kThreads) k_spread_count(LatticeGeom g, MarkerArrays m, SpreadLists sl) {
+  const int lane = threadIdx.x & 63;
+  const int l = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  if (l >= m.M) return;
+  const StencilLane st = stencil_lane(g, m.lag[l], lane);
+  float ws = 0.f;
+#pragma unroll
+  for (int j = 0; j < 64; ++j) ws += lane_read(st.w, j);
   const size_t k = (size_t)l * 64 + lane;
-  if (st.cell < 0 || scale < 0.0f) {
-    ctab[k] = -1;
+  if (lane == 0) sl.wsum[l] = ws;
+  if (st.cell < 0 || ws == 0.0f) {
+    sl.ctab[k] = -1;
+    sl.rtab[k] = 0.f;
     return;
   }
-  ctab[k] = st.cell;
-  wtab[k] = st.w * scale;
+  const float inv_h3 = (float)(1.0 / (double)(g.h * g.h * g.h));
+  const float scale = 1 * inv_h3 / ws;
+  sl.ctab[k] = st.cell;
+  sl.rtab[k] = st.w;
+  sl.wtab[k] = st.w * scale;
   atomicAdd(&sl.cnt[st.cell], 1);
   sl.blkflag[st.cell / kFlagCells] = 1;
 }
@@ -247,26 +267,9 @@ __global__ void k_spread_clear_flags(SpreadLists sl) {
   if (b < sl.nblk) sl.blkflag[b] = 0;
 }
 
-__device__ inline int block_excl_scan(int v, int* s_tmp, int& total) {
-  // exclusive scan over kThreads values (Hillis-Steele in LDS)
-  const int t = threadIdx.x;
-  s_tmp[t] = v;
-  __syncthreads();
-  for (int o = 1; o < kThreads; o <<= 1) {
-    const int a = t >= o ? s_tmp[t - o] : 0;
-    __syncthreads();
-    s_tmp[t] += a;
-    __syncthreads();
-  }
-  total = s_tmp[kThreads - 1];
-  const int incl = s_tmp[t];
-  __syncthreads();
-  return incl - v;
-}
-
 // per flagged block: total contributions
 __global__ void __launch_bounds__(kThreads) k_spread_block_sum(LatticeGeom g, SpreadLists sl) {
-  __shared__ int s_tmp[kThreads];
+  __shared__ int s_wave[kWavesPerBlock];
   const int b = blockIdx.x;
   if (!sl.blkflag[b]) {
     if (threadIdx.x == 0) sl.bsum[b] = 0;
@@ -275,55 +278,61 @@ __global__ void __launch_bounds__(kThreads) k_spread_block_sum(LatticeGeom g, Sp
   const int i = b * kFlagCells + threadIdx.x;
   const int v = i < g.N ? sl.cnt[i] : 0;
   int total;
-  block_excl_scan(v, s_tmp, total);
+  block_excl_scan(v, s_wave, total);
   if (threadIdx.x == 0) sl.bsum[b] = total;
 }
 
-// exclusive scan of the block totals (one work-group); bsum[nblk] = total
+// exclusive scan of the block totals (one work-group, tiles of 256 x 8
+// entries); bsum[nblk] = total
 __global__ void __launch_bounds__(kThreads) k_spread_scan_blocks(SpreadLists sl) {
-  __shared__ int s_tmp[kThreads];
-  __shared__ int s_carry;
-  if (threadIdx.x == 0) s_carry = 0;
-  __syncthreads();
-  for (int base = 0; base < sl.nblk; base += kThreads) {
-    const int b = base + threadIdx.x;
-    const int v = b < sl.nblk ? sl.bsum[b] : 0;
+  __shared__ int s_wave[kWavesPerBlock];
+  constexpr int kPer = 8;
+  int carry = 0;
+  for (int base = 0; base < sl.nblk; base += kThreads * kPer) {
+    const int b0 = base + threadIdx.x * kPer;
+    int v[kPer];
+    int sum = 0;
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+      v[j] = b0 + j < sl.nblk ? sl.bsum[b0 + j] : 0;
+      sum += v[j];
+    }
     int total;
-    const int ex = block_excl_scan(v, s_tmp, total);
-    const int carry = s_carry;
-    if (b < sl.nblk) sl.bsum[b] = carry + ex;
-    __syncthreads();
-    if (threadIdx.x == 0) s_carry = carry + total;
-    __syncthreads();
+    int run = carry + block_excl_scan(sum, s_wave, total);
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+      if (b0 + j < sl.nblk) sl.bsum[b0 + j] = run;
+      run += v[j];
+    }
+    carry += total;
   }
-  if (threadIdx.x == 0) sl.bsum[sl.nblk] = s_carry;
+  if (threadIdx.x == 0) sl.bsum[sl.nblk] = carry;
 }
 
 // per flagged block: each cell's first entry and count; counts reset to 0
 __global__ void __launch_bounds__(kThreads) k_spread_local(LatticeGeom g, SpreadLists sl) {
-  __shared__ int s_tmp[kThreads];
+  __shared__ int s_wave[kWavesPerBlock];
   const int b = blockIdx.x;
   if (!sl.blkflag[b]) return;
   const int i = b * kFlagCells + threadIdx.x;
   const int v = i < g.N ? sl.cnt[i] : 0;
   int total;
-  const int ex = block_excl_scan(v, s_tmp, total);
+  const int ex = block_excl_scan(v, s_wave, total);
   if (i < g.N) {
     sl.rng[i] = make_int2(sl.bsum[b] + ex, v);
     sl.cnt[i] = 0;
   }
 }
 
-__global__ void __launch_bounds__(kThreads) k_spread_fill(int M64, SpreadLists sl, const int* __restrict__ ctab,
-                                                          const float* __restrict__ wtab) {
+__global__ void __launch_bounds__(kThreads) k_spread_fill(int M64, SpreadLists sl) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= M64) return;
-  const int cell = ctab[k];
+  const int cell = sl.ctab[k];
   if (cell < 0) return;
   const int pos = sl.rng[cell].x + atomicAdd(&sl.fillpos[cell], 1);
   SpreadEntry e;
   e.marker = k >> 6;
-  e.w = wtab[k];
+  e.w = sl.wtab[k];
   sl.ent[pos] = e;
 }
 
@@ -420,30 +429,28 @@ void launch_scatter(const MarkerArrays& m, hipStream_t s) {
 }
 void launch_repulsion(const MarkerArrays& m, float delta, float gamma, hipStream_t s) {
   if (m.M == 0) return;
-  k_repulsion<<<blocks_for(m.M, kThreads), kThreads, 0, s>>>(m, delta, gamma);
+  k_repulsion<<<blocks_for(m.M, kWavesPerBlock), kThreads, 0, s>>>(m, delta, gamma);
   GLSM_LAUNCH_CHECK();
 }
-void launch_ibm_interp(const LatticeGeom& g, const MarkerArrays& m, const float3* u, float beta_eff,
+void launch_ibm_interp(const MarkerArrays& m, const SpreadLists& sl, const float3* u, float beta_eff,
                        hipStream_t s) {
   if (m.M == 0) return;
-  k_ibm_interp<<<blocks_for(m.M, kWavesPerBlock), kThreads, 0, s>>>(g, m, u, beta_eff);
+  k_ibm_interp<<<blocks_for(m.M, kWavesPerBlock), kThreads, 0, s>>>(m, sl, u, beta_eff);
   GLSM_LAUNCH_CHECK();
 }
-void launch_ibm_sample(const LatticeGeom& g, const MarkerArrays& m, const float* c, hipStream_t s) {
+void launch_ibm_sample(const MarkerArrays& m, const SpreadLists& sl, const float* c, hipStream_t s) {
   if (m.M == 0) return;
-  k_ibm_sample<<<blocks_for(m.M, kWavesPerBlock), kThreads, 0, s>>>(g, m, c);
+  k_ibm_sample<<<blocks_for(m.M, kWavesPerBlock), kThreads, 0, s>>>(m, sl, c);
   GLSM_LAUNCH_CHECK();
 }
 
 void launch_spread_rebuild(const LatticeGeom& g, const MarkerArrays& m, const SpreadLists& sl, hipStream_t s) {
-  int* ctab = sl.ctab;
-  float* wtab = sl.wtab;
   k_spread_clear_flags<<<blocks_for(sl.nblk, kThreads), kThreads, 0, s>>>(sl);
-  if (m.M > 0) k_spread_count<<<blocks_for(m.M, kWavesPerBlock), kThreads, 0, s>>>(g, m, sl, ctab, wtab);
+  if (m.M > 0) k_spread_count<<<blocks_for(m.M, kWavesPerBlock), kThreads, 0, s>>>(g, m, sl);
   k_spread_block_sum<<<sl.nblk, kThreads, 0, s>>>(g, sl);
   k_spread_scan_blocks<<<1, kThreads, 0, s>>>(sl);
   k_spread_local<<<sl.nblk, kThreads, 0, s>>>(g, sl);
-  if (m.M > 0) k_spread_fill<<<blocks_for((long)m.M * 64, kThreads), kThreads, 0, s>>>(m.M * 64, sl, ctab, wtab);
+  if (m.M > 0) k_spread_fill<<<blocks_for((long)m.M * 64, kThreads), kThreads, 0, s>>>(m.M * 64, sl);
   k_spread_sort<<<sl.nblk, kThreads, 0, s>>>(g, sl);
   GLSM_LAUNCH_CHECK();
 }

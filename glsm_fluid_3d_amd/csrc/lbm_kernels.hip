@@ -43,40 +43,53 @@ constexpr int CZ[19] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, 1, -1, -1, 1, -1, -
 constexpr int OPP[19] = {0, 2, 1, 4, 3, 6, 5, 9, 10, 7, 8, 13, 14, 11, 12, 17, 18, 15, 16};  // fluid.cu:215
 
 // ---- storage policies ----
+// Loads are plain (cached): non-temporal loads measured ~11% slower on
+// gfx950.  Stores are non-temporal only when the two distribution buffers
+// exceed the 256 MB Infinity Cache (NT): below that, plain stores leave the
+// next sub-step's input in the cache.
+template <class T, bool NT>
+__device__ __forceinline__ void put(T* p, T v) {
+  if (NT)
+    __builtin_nontemporal_store(v, p);
+  else
+    *p = v;
+}
 template <int S>
 struct Store;
 template <>
 struct Store<F32_FULL> {
   using T = float;
   static constexpr bool dev = false;
-  __device__ static float ld(const T* p, float) { return __builtin_nontemporal_load(p); }
-  __device__ static void st(T* p, float v, float) { __builtin_nontemporal_store(v, p); }
+  __device__ static float ld(const T* p, float) { return *p; }
+  template <bool NT>
+  __device__ static void st(T* p, float v, float) { put<T, NT>(p, v); }
 };
 template <>
 struct Store<F32_DEV> {
   using T = float;
   static constexpr bool dev = true;
-  __device__ static float ld(const T* p, float) { return __builtin_nontemporal_load(p); }
-  __device__ static void st(T* p, float v, float) { __builtin_nontemporal_store(v, p); }
+  __device__ static float ld(const T* p, float) { return *p; }
+  template <bool NT>
+  __device__ static void st(T* p, float v, float) { put<T, NT>(p, v); }
 };
 template <>
 struct Store<F16_DEV> {
   using T = _Float16;
   static constexpr bool dev = true;
-  __device__ static float ld(const T* p, float inv_scale) { return (float)__builtin_nontemporal_load(p) * inv_scale; }
-  __device__ static void st(T* p, float v, float scale) { __builtin_nontemporal_store((_Float16)(v * scale), p); }
+  __device__ static float ld(const T* p, float inv_scale) { return (float)*p * inv_scale; }
+  template <bool NT>
+  __device__ static void st(T* p, float v, float scale) { put<T, NT>(p, (_Float16)(v * scale)); }
 };
 template <>
 struct Store<BF16_DEV> {
   using T = uint16_t;
   static constexpr bool dev = true;
-  __device__ static float ld(const T* p, float) {
-    return __uint_as_float((uint32_t)__builtin_nontemporal_load(p) << 16);
-  }
+  __device__ static float ld(const T* p, float) { return __uint_as_float((uint32_t)*p << 16); }
+  template <bool NT>
   __device__ static void st(T* p, float v, float) {
     const uint32_t b = __float_as_uint(v);
     const uint32_t r = (b + 0x7FFFu + ((b >> 16) & 1u)) >> 16;  // round to nearest even (finite v)
-    __builtin_nontemporal_store((uint16_t)r, p);
+    put<T, NT>(p, (uint16_t)r);
   }
 };
 
@@ -155,7 +168,7 @@ __device__ __forceinline__ float wq(const LbmParams& p, int q) { return q == 0 ?
 
 // BGK + Guo on one cell.  v[] holds f (full storage) or d = f - w (deviation
 // storage); the same representation is written back.
-template <int S, bool FORCED>
+template <int S, bool FORCED, bool NT>
 __device__ __forceinline__ void collide_store(const float (&v)[19], float Fx, float Fy, float Fz, const LbmParams& p,
                                               typename Store<S>::T* __restrict__ fout, unsigned plane, unsigned i,
                                               float3* u, bool store_u) {
@@ -196,11 +209,11 @@ __device__ __forceinline__ void collide_store(const float (&v)[19], float Fx, fl
       const float ciF = cx * Fx + cy * Fy + cz * Fz;
       out += w * kF * (3.0f * (ciF - uF) + 9.0f * cu * ciF);
     }
-    Store<S>::st(fout + (unsigned)q * plane + i, out, p.scale);
+    Store<S>::template st<NT>(fout + (unsigned)q * plane + i, out, p.scale);
   });
 }
 
-template <int S>
+template <int S, bool NT>
 __global__ void __launch_bounds__(kThreads) k_lbm_step(LatticeGeom g, LbmParams p,
                                                        const typename Store<S>::T* __restrict__ fin,
                                                        typename Store<S>::T* __restrict__ fout, unsigned plane,
@@ -231,9 +244,9 @@ __global__ void __launch_bounds__(kThreads) k_lbm_step(LatticeGeom g, LbmParams 
         Fz += fl.z * e.w;
       }
     }
-    collide_store<S, true>(v, p.Fc.x + Fx, p.Fc.y + Fy, p.Fc.z + Fz, p, fout, plane, i, u, store_u);
+    collide_store<S, true, NT>(v, p.Fc.x + Fx, p.Fc.y + Fy, p.Fc.z + Fz, p, fout, plane, i, u, store_u);
   } else {
-    collide_store<S, false>(v, 0.f, 0.f, 0.f, p, fout, plane, i, u, store_u);
+    collide_store<S, false, NT>(v, 0.f, 0.f, 0.f, p, fout, plane, i, u, store_u);
   }
 }
 
@@ -247,7 +260,8 @@ __global__ void __launch_bounds__(kThreads) k_lbm_init(LatticeGeom g, LbmParams 
   const unsigned i = blockIdx.x * kThreads + threadIdx.x;
   if (i >= (unsigned)g.N) return;
 #pragma unroll
-  for (int q = 0; q < 19; ++q) Store<S>::st(f + (unsigned)q * plane + i, Store<S>::dev ? 0.f : wq(p, q), p.scale);
+  for (int q = 0; q < 19; ++q)
+    Store<S>::template st<false>(f + (unsigned)q * plane + i, Store<S>::dev ? 0.f : wq(p, q), p.scale);
   u[i] = make_float3(0.f, 0.f, 0.f);
   c1[i] = 0.f;
   c2[i] = 0.f;
@@ -286,7 +300,7 @@ __global__ void __launch_bounds__(kThreads) k_lbm_import(LatticeGeom g, LbmParam
   Unroll<19>::run([&](auto qc) {
     constexpr int q = decltype(qc)::value;
     const float v = in[a.dst<q>(i, in_plane)];  // from plane q or opp q; w_opp(q) == w_q
-    Store<S>::st(fpost + (unsigned)q * plane + i, Store<S>::dev ? v - wq(p, q) : v, p.scale);
+    Store<S>::template st<false>(fpost + (unsigned)q * plane + i, Store<S>::dev ? v - wq(p, q) : v, p.scale);
   });
 }
 
@@ -297,8 +311,17 @@ void step_impl(const LatticeGeom& g, const LbmParams& p, const void* fin, void* 
                bool write_u_all, const SpreadLists& sl, const float3* Fl, hipStream_t s) {
   using T = typename Store<S>::T;
   const int const_force = (p.Fc.x != 0.f || p.Fc.y != 0.f || p.Fc.z != 0.f) ? 1 : 0;
-  k_lbm_step<S><<<blocks_for(g.N), kThreads, 0, s>>>(g, p, static_cast<const T*>(fin), static_cast<T*>(fout),
-                                                     (unsigned)plane, u, write_u_all ? 1 : 0, sl, Fl, const_force);
+  // both ping-pong buffers larger than ~3/4 of the 256 MB Infinity Cache:
+  // stream the stores past it
+  const bool nt = 2.0 * 19.0 * (double)plane * sizeof(T) > 192.0 * (1 << 20);
+  if (nt)
+    k_lbm_step<S, true><<<blocks_for(g.N), kThreads, 0, s>>>(g, p, static_cast<const T*>(fin), static_cast<T*>(fout),
+                                                             (unsigned)plane, u, write_u_all ? 1 : 0, sl, Fl,
+                                                             const_force);
+  else
+    k_lbm_step<S, false><<<blocks_for(g.N), kThreads, 0, s>>>(g, p, static_cast<const T*>(fin),
+                                                              static_cast<T*>(fout), (unsigned)plane, u,
+                                                              write_u_all ? 1 : 0, sl, Fl, const_force);
 }
 
 }  // namespace

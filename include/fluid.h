@@ -140,7 +140,9 @@ class Fluid {
   int* d_bsum_ = nullptr;     // per-block entry totals -> offsets
   SpreadEntry* d_ent_ = nullptr;
   int* d_ctab_ = nullptr;     // per (marker, stencil lane): cell or -1
-  float* d_wtab_ = nullptr;   // per (marker, stencil lane): weight
+  float* d_wtab_ = nullptr;   // per (marker, stencil lane): spreading weight
+  float* d_rtab_ = nullptr;   // per (marker, stencil lane): interpolation weight
+  float* d_wsum_ = nullptr;   // per marker: sum of interpolation weights
   int ent_cap_ = 0;
   uint64_t built_lag_version_ = ~0ull;
 };
