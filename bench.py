@@ -139,8 +139,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--extra-steps", type=int, default=400)
-    ap.add_argument("--workload", default="all", choices=["all", "101", "201-f32", "201-f16", "201-bf16"],
-                    help="profile one workload alone (no CPU baseline)")
+    ap.add_argument("--workload", default="all", choices=["all", "101", "201-f32", "201-f16", "201-bf16", "copy"],
+                    help="profile one workload alone (no CPU baseline); 'copy' = the 1.23 GB calibration copy")
     args = ap.parse_args()
     if args.workload != "all":
         args.no_cpu_baseline = True
@@ -153,6 +153,11 @@ def main():
     import glsm_fluid_3d_amd as G
 
     G.init(0)
+    if args.workload == "copy":
+        nbytes = 2 * 19 * 4 * 8120601 // 2  # one 201^3 FP32 distribution buffer
+        print(json.dumps({"workload": "plain 16-B copy", "bytes": nbytes,
+                          "GBs": round(G.copy_bandwidth(nbytes, max(1, args.steps)), 1)}), flush=True)
+        return
     if args.workload.startswith("201"):
         st = args.workload.split("-")[1]
         gels, c, f = config2b(G, st)
@@ -184,6 +189,9 @@ def main():
     log(f"101^3: {mlups:.0f} MLUPS, {dt / args.steps * 1e3:.3f} ms/step, lbm kernel {lbm_ms * 1e3:.1f} us")
     for x in gels + [c, f]:
         x.close()
+    copy_gbs = G.copy_bandwidth(19 * 4 * 8120601, 20)
+    line["roofline"]["copy_GBs_1.23GB"] = round(copy_gbs, 1)
+    log(f"plain copy of 1.23 GB: {copy_gbs:.0f} GB/s")
 
     if not args.no_extra:
         extra = {}

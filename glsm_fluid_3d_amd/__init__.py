@@ -80,6 +80,7 @@ SIGNATURES = {
     "glsm_fluid_read": (_i, [_vp, _i, _vp, _sz]),
     "glsm_fluid_write": (_i, [_vp, _i, _vp, _sz]),
     "glsm_system_step": (_i, [C.POINTER(_vp), _i, _vp, _vp, _i]),
+    "glsm_copy_bandwidth": (_i, [_sz, _i, C.POINTER(_d)]),
 }
 
 _lib = None
@@ -317,6 +318,13 @@ class Fluid:
         shape = {"u": (self.N, 3), "c": (self.N,), "f": (19, self.N)}[name]
         a = np.ascontiguousarray(np.asarray(arr, dtype=np.float32).reshape(shape))
         _check(load().glsm_fluid_write(self.h, FL_FIELD[name], a.ctypes.data, a.nbytes), f"fluid write {name}")
+
+
+def copy_bandwidth(nbytes: int, reps: int = 20) -> float:
+    """GB/s ((read + written) bytes / s) of a plain 16-byte-per-lane device copy."""
+    g = C.c_double()
+    _check(load().glsm_copy_bandwidth(int(nbytes), int(reps), C.byref(g)), "copy_bandwidth")
+    return g.value
 
 
 def system_step(gels, coupler: Coupler, fluid: Fluid, it: int):

@@ -400,6 +400,33 @@ int glsm_fluid_write(glsm_fluid* f, int field, const void* host, size_t bytes) {
   });
 }
 
+// ---------------------------------------------------------------- measurement
+int glsm_copy_bandwidth(size_t bytes, int reps, double* gbps) {
+  return guarded([&] {
+    REQUIRE(gbps && reps > 0 && bytes >= 16, "bad copy benchmark arguments");
+    bytes = bytes / 16 * 16;
+    hipStream_t s = glsm::stream();
+    char* a = glsm::dalloc<char>(bytes);
+    char* b = glsm::dalloc<char>(bytes);
+    hipEvent_t e0, e1;
+    GLSM_HIP(hipEventCreate(&e0));
+    GLSM_HIP(hipEventCreate(&e1));
+    glsm::launch_copy16(a, b, bytes, s);
+    GLSM_HIP(hipEventRecord(e0, s));
+    for (int r = 0; r < reps; ++r) glsm::launch_copy16(r % 2 ? b : a, r % 2 ? a : b, bytes, s);
+    GLSM_HIP(hipEventRecord(e1, s));
+    GLSM_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    GLSM_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    glsm::dfree(a);
+    glsm::dfree(b);
+    *gbps = 2.0 * (double)bytes * reps / (ms * 1e-3) / 1e9;
+    return GLSM_OK;
+  });
+}
+
 // ---------------------------------------------------------------- system
 int glsm_system_step(glsm_gel* const* gels, int n, glsm_coupler* c, glsm_fluid* f, int iter) {
   return guarded([&] {

@@ -139,6 +139,8 @@ void launch_ibm_sample(const MarkerArrays& m, const SpreadLists& sl, const float
 // distribution storage representations (collision is FP32 in all of them)
 enum StorageMode { F32_FULL = 0, F32_DEV = 1, F16_DEV = 2, BF16_DEV = 3 };
 size_t storage_bytes(int storage);
+// device copy of `bytes` (multiple of 16), 16 B per lane
+void launch_copy16(const void* a, void* b, size_t bytes, hipStream_t s);
 
 struct LbmParams {
   float omega;     // 1/tau

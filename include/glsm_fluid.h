@@ -156,6 +156,12 @@ size_t glsm_fluid_field_bytes(glsm_fluid* f, int field);
 int glsm_fluid_read(glsm_fluid* f, int field, void* host, size_t bytes);
 int glsm_fluid_write(glsm_fluid* f, int field, const void* host, size_t bytes);
 
+/* Extension (measurement, no reference counterpart): bandwidth of a plain
+   16-B-per-lane device copy of `bytes` bytes, `reps` times; reported as
+   (bytes read + bytes written) / s in GB/s.  The reference point for the
+   roofline and for calibrating HBM traffic counters. */
+int glsm_copy_bandwidth(size_t bytes, int reps, double* gbps);
+
 /* ---- one time step in sim.cpp's order (sim.cpp:66-81): for each gel
    stepElasticity, stepChemistry, recordCenter; packFromGels; stepVelocity;
    stepConcentration; applyGelRepulsion; scatterToGels.  The gels must be the
