@@ -75,6 +75,95 @@ def config2b(mod, storage):
     return gels, c, f
 
 
+def round_to_odd(x):
+    return int(2 * round((x - 1) / 2) + 1)
+
+
+def place_gels(rng, box, sizes, margin=2.0, tries=200):
+    """Seeded centres for cube gels of the given element counts inside a
+    cubic box, at least `margin` from the walls and one element apart from
+    each other (boxes separated along some axis); gels that find no place in
+    `tries` draws are dropped."""
+    out = []
+    for n in sizes:
+        half = n / 2.0
+        lo, hi = half + margin, box - half - margin
+        if hi <= lo:
+            continue
+        for _ in range(tries):
+            p = rng.uniform(lo, hi, 3)
+            if all(np.any(np.abs(p - q) >= half + m / 2.0 + 1.0) for q, m in out):
+                out.append((p, n))
+                break
+    return out
+
+
+def batch_specs(count=64, seed=3, edge_range=(33, 97), gels_range=(1, 4), n_range=(4, 8), rate_jitter=0.2):
+    """Independent instances of the batching workload: lattice edge
+    log-uniform in edge_range rounded to odd (box = (edge - 1) h, h = 0.5),
+    1-4 cube gels of 4^3-8^3 elements at seeded non-overlapping positions,
+    rate constants f, ep uniform within +-20% of 0.9, 0.3."""
+    rng = np.random.default_rng(seed)
+    specs = []
+    for k in range(count):
+        edge = round_to_odd(np.exp(rng.uniform(np.log(edge_range[0]), np.log(edge_range[1]))))
+        box = (edge - 1) * 0.5
+        ng = int(rng.integers(gels_range[0], gels_range[1] + 1))
+        sizes = [int(rng.integers(n_range[0], n_range[1] + 1)) for _ in range(ng)]
+        placed = place_gels(rng, box, sizes)
+        gels = [(n, tuple(float(v) for v in p), 0.9 * rng.uniform(1 - rate_jitter, 1 + rate_jitter),
+                 0.3 * rng.uniform(1 - rate_jitter, 1 + rate_jitter)) for p, n in placed]
+        specs.append({"edge": edge, "box": box, "gels": gels, "seed": seed * 1000 + k})
+    return specs
+
+
+def build_instance(mod, spec, storage=None):
+    gels = []
+    for j, (n, pos, f, ep) in enumerate(spec["gels"]):
+        g = mod.Gel((n, n, n), pos, gel_id=j + 1)
+        g.set_chemistry_rates(f, ep)
+        g.perturb_chemistry(spec["seed"] * 8 + j, 0.01)
+        gels.append(g)
+    c = mod.Coupler(gels)
+    b = spec["box"]
+    f = mod.Fluid((b, b, b), c)
+    if storage:
+        f.set_storage(storage)
+    return gels, c, f
+
+
+def run_batch(G, storage, steps, warmup, count=64, tune=True):
+    specs = batch_specs(count)
+    insts = [build_instance(G, sp, storage) for sp in specs]
+    b = G.Batch(storage)
+    for inst in insts:
+        b.add(*inst)
+    b.step(1)
+    shape, tuned_ms = b.autotune(reps=3) if tune else (b.launch(), None)
+    b.step(max(0, warmup - 1))
+    b.enable_lbm_timing(10)
+    G.synchronize()
+    t0 = time.perf_counter()
+    b.step(steps)
+    G.synchronize()
+    dt = time.perf_counter() - t0
+    lbm_ms, samples = b.lbm_timing()
+    cells = sum(f.N for _, _, f in insts)
+    nsub = insts[0][2].nsub
+    out = {"value": round(cells * nsub * steps / dt / 1e6, 1), "unit": "MLUPS", "steps": steps,
+           "ms_per_step": round(dt / steps * 1e3, 4), "instances": count, "cells": cells,
+           "gels": sum(len(g) for g, _, _ in insts), "markers": sum(c.M for _, c, _ in insts),
+           "launch": shape, "hip_graph": True,
+           "roofline": roofline(cells, 2 * 19 * G.STORAGE_BYTES[storage], lbm_ms)}
+    out["roofline"]["kernel"] = "kb_lbm_step (batched persistent pull stream + collide)"
+    out["roofline"]["samples"] = samples
+    b.close()
+    for gels, c, f in insts:
+        for x in gels + [f, c]:
+            x.close()
+    return out
+
+
 def run_gpu(G, gels, c, f, steps, warmup, every=10):
     for it in range(warmup):
         G.system_step(gels, c, f, it)
@@ -139,7 +228,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--extra-steps", type=int, default=400)
-    ap.add_argument("--workload", default="all", choices=["all", "101", "201-f32", "201-f16", "201-bf16", "copy"],
+    ap.add_argument("--batch-steps", type=int, default=300)
+    ap.add_argument("--workload", default="all",
+                    choices=["all", "101", "201-f32", "201-f16", "201-bf16", "copy", "batch-f32", "batch-f16"],
                     help="profile one workload alone (no CPU baseline); 'copy' = the 1.23 GB calibration copy")
     args = ap.parse_args()
     if args.workload != "all":
@@ -157,6 +248,12 @@ def main():
         nbytes = 2 * 19 * 4 * 8120601 // 2  # one 201^3 FP32 distribution buffer
         print(json.dumps({"workload": "plain 16-B copy", "bytes": nbytes,
                           "GBs": round(G.copy_bandwidth(nbytes, max(1, args.steps)), 1)}), flush=True)
+        return
+    if args.workload.startswith("batch"):
+        st = args.workload.split("-")[1]
+        r = run_batch(G, st, args.steps, args.warmup)
+        r["workload"] = f"64 instances (edge 33-97, 1-4 gels of 4^3-8^3), {st}"
+        print(json.dumps(r), flush=True)
         return
     if args.workload.startswith("201"):
         st = args.workload.split("-")[1]
@@ -206,6 +303,11 @@ def main():
             log(f"201^3 {st}: {v2:.0f} MLUPS, lbm kernel {ms2 * 1e3:.1f} us")
             for x in gels + [c, f]:
                 x.close()
+        for st in ("f32", "f16"):
+            r = run_batch(G, st, args.batch_steps, 20)
+            extra[f"batch64_{st}"] = r
+            log(f"batch of 64 {st}: {r['value']:.0f} MLUPS, lbm kernel {r['roofline']['mean_launch_ms'] * 1e3:.1f} us, "
+                f"launch {r['launch']}")
         line["extra"] = extra
 
     if not args.no_cpu_baseline:

@@ -46,6 +46,8 @@ SIGNATURES = {
     "glsm_gel_step_chemistry": (_i, [_vp, _i]),
     "glsm_gel_record_center": (_i, [_vp, _i]),
     "glsm_gel_write_files": (_i, [_vp, _i]),
+    "glsm_gel_save_state": (_i, [_vp, C.c_char_p]),
+    "glsm_gel_load_state": (_i, [_vp, C.c_char_p]),
     "glsm_gel_reset_to_quiescent": (_i, [_vp]),
     "glsm_gel_fire_excitation_pulse": (_i, [_vp]),
     "glsm_gel_set_anchor_z": (_i, [_vp, _d]),
@@ -53,6 +55,7 @@ SIGNATURES = {
     "glsm_gel_get_index": (_i, [_vp, _i, _i, _i, _i]),
     "glsm_gel_dims": (_i, [_vp, _pi, _pi, _pi, _pi]),
     "glsm_gel_perturb_chemistry": (_i, [_vp, C.c_uint64, _d]),
+    "glsm_gel_set_chemistry_rates": (_i, [_vp, _d, _d]),
     "glsm_gel_field_bytes": (_sz, [_vp, _i]),
     "glsm_gel_read": (_i, [_vp, _i, _vp, _sz]),
     "glsm_gel_write": (_i, [_vp, _i, _vp, _sz]),
@@ -70,6 +73,9 @@ SIGNATURES = {
     "glsm_fluid_step_concentration": (_i, [_vp, _i]),
     "glsm_fluid_sub_step": (_i, [_vp, _f]),
     "glsm_fluid_write_files": (_i, [_vp, _i]),
+    "glsm_fluid_save_state": (_i, [_vp, C.c_char_p]),
+    "glsm_fluid_conservation": (_i, [_vp, C.POINTER(_d)]),
+    "glsm_fluid_load_state": (_i, [_vp, C.c_char_p]),
     "glsm_fluid_set_periodic": (_i, [_vp, _i, _i, _i]),
     "glsm_fluid_set_ibm": (_i, [_vp, _f, _f]),
     "glsm_fluid_set_storage": (_i, [_vp, _i, _i]),
@@ -79,6 +85,19 @@ SIGNATURES = {
     "glsm_fluid_field_bytes": (_sz, [_vp, _i]),
     "glsm_fluid_read": (_i, [_vp, _i, _vp, _sz]),
     "glsm_fluid_write": (_i, [_vp, _i, _vp, _sz]),
+    "glsm_batch_create": (_i, [_i, _i, C.POINTER(_vp)]),
+    "glsm_batch_destroy": (_i, [_vp]),
+    "glsm_batch_add": (_i, [_vp, _vp, _vp, _i, _i]),
+    "glsm_batch_remove": (_i, [_vp, _i]),
+    "glsm_batch_size": (_i, [_vp]),
+    "glsm_batch_iteration": (_i, [_vp, _i]),
+    "glsm_batch_step": (_i, [_vp, _i]),
+    "glsm_batch_use_graph": (_i, [_vp, _i]),
+    "glsm_batch_set_launch": (_i, [_vp, _i, _i, _i]),
+    "glsm_batch_get_launch": (_i, [_vp, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i)]),
+    "glsm_batch_autotune": (_i, [_vp, _i, C.POINTER(_d)]),
+    "glsm_batch_enable_lbm_timing": (_i, [_vp, _i]),
+    "glsm_batch_lbm_timing": (_i, [_vp, C.POINTER(_d), C.POINTER(_i)]),
     "glsm_system_step": (_i, [C.POINTER(_vp), _i, _vp, _vp, _i]),
     "glsm_copy_bandwidth": (_i, [_sz, _i, C.POINTER(_d)]),
 }
@@ -189,6 +208,13 @@ class Gel:
     def write_files(self, it):
         _check(load().glsm_gel_write_files(self.h, int(it)), "write_files")
 
+    def save_state(self, path):
+        """Lossless binary snapshot (every device field, params, flags)."""
+        _check(load().glsm_gel_save_state(self.h, os.fsencode(path)), "save_state")
+
+    def load_state(self, path):
+        _check(load().glsm_gel_load_state(self.h, os.fsencode(path)), "load_state")
+
     def reset_to_quiescent(self):
         _check(load().glsm_gel_reset_to_quiescent(self.h), "reset_to_quiescent")
 
@@ -203,6 +229,10 @@ class Gel:
 
     def perturb_chemistry(self, seed, amplitude):
         _check(load().glsm_gel_perturb_chemistry(self.h, int(seed), float(amplitude)), "perturb_chemistry")
+
+    def set_chemistry_rates(self, f, ep):
+        """Rate constants f, ep; steady state solved again, chemistry reset to it."""
+        _check(load().glsm_gel_set_chemistry_rates(self.h, float(f), float(ep)), "set_chemistry_rates")
 
     def get_index(self, xi, yi, zi, size):
         return _check(load().glsm_gel_get_index(self.h, xi, yi, zi, size), "get_index")
@@ -288,6 +318,20 @@ class Fluid:
     def write_files(self, it):
         _check(load().glsm_fluid_write_files(self.h, int(it)), "write_files")
 
+    def conservation(self):
+        """FP64 (sum(rho - 1), sum(rho ux), sum(rho uy), sum(rho uz)) of the
+        pre-collision populations of the next sub-step."""
+        out = (_d * 4)()
+        _check(load().glsm_fluid_conservation(self.h, out), "conservation")
+        return np.array(out[:], dtype=np.float64)
+
+    def save_state(self, path):
+        """Lossless binary snapshot (stored populations, u, c, params)."""
+        _check(load().glsm_fluid_save_state(self.h, os.fsencode(path)), "save_state")
+
+    def load_state(self, path):
+        _check(load().glsm_fluid_load_state(self.h, os.fsencode(path)), "load_state")
+
     def set_periodic(self, px, py, pz):
         _check(load().glsm_fluid_set_periodic(self.h, int(px), int(py), int(pz)), "set_periodic")
 
@@ -331,3 +375,67 @@ def system_step(gels, coupler: Coupler, fluid: Fluid, it: int):
     """One time step in sim.cpp's order (sim.cpp:66-81)."""
     arr = (C.c_void_p * len(gels))(*[g.h for g in gels])
     _check(load().glsm_system_step(arr, len(gels), coupler.h, fluid.h, int(it)), "glsm_system_step")
+
+
+class Batch:
+    """Many gel-fluid instances advanced together (include/batch.h).
+
+    Each member is (gels, coupler, fluid); the Python objects must stay
+    alive while they are members.  step() advances every member by one time
+    step with one launch per phase (HIP-graph captured); each member gives
+    the same bits as system_step on its own objects."""
+
+    def __init__(self, storage="f32", scale_exp=20):
+        h = C.c_void_p()
+        _check(load().glsm_batch_create(STORAGE[storage], int(scale_exp), C.byref(h)), "glsm_batch_create")
+        self.h = h.value
+        self.members = {}
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().glsm_batch_destroy(self.h)
+            self.h = None
+            self.members = {}
+
+    def add(self, gels, coupler, fluid, iteration=0, write_files=False):
+        mid = _check(load().glsm_batch_add(self.h, coupler.h, fluid.h, int(iteration), int(bool(write_files))),
+                     "batch add")
+        self.members[mid] = (gels, coupler, fluid)
+        return mid
+
+    def remove(self, mid):
+        _check(load().glsm_batch_remove(self.h, int(mid)), "batch remove")
+        self.members.pop(mid, None)
+
+    def __len__(self):
+        return _check(load().glsm_batch_size(self.h), "batch size")
+
+    def iteration(self, mid):
+        return _check(load().glsm_batch_iteration(self.h, int(mid)), "batch iteration")
+
+    def step(self, n=1):
+        _check(load().glsm_batch_step(self.h, int(n)), "batch step")
+
+    def use_graph(self, on):
+        _check(load().glsm_batch_use_graph(self.h, int(bool(on))), "batch use_graph")
+
+    def set_launch(self, nt, run, grid):
+        _check(load().glsm_batch_set_launch(self.h, int(nt), int(run), int(grid)), "batch set_launch")
+
+    def launch(self):
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        _check(load().glsm_batch_get_launch(self.h, C.byref(a), C.byref(b), C.byref(c)), "batch launch")
+        return {"nt": a.value, "run": b.value, "grid": c.value}
+
+    def autotune(self, reps=5):
+        ms = C.c_double()
+        _check(load().glsm_batch_autotune(self.h, int(reps), C.byref(ms)), "batch autotune")
+        return self.launch(), ms.value
+
+    def enable_lbm_timing(self, every):
+        _check(load().glsm_batch_enable_lbm_timing(self.h, int(every)), "batch enable_lbm_timing")
+
+    def lbm_timing(self):
+        ms, n = C.c_double(), C.c_int()
+        _check(load().glsm_batch_lbm_timing(self.h, C.byref(ms), C.byref(n)), "batch lbm_timing")
+        return ms.value, n.value

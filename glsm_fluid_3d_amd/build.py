@@ -4,9 +4,10 @@
 csrc/ with hipcc (one object per source, in parallel) and links
 glsm_fluid_3d_amd/libglsm_fluid.so.  Kernels that must follow the
 reference's arithmetic rounding step by step (gel FP64, IBM, scalar,
-coupling) are built with -ffp-contract=off; the LBM kernel keeps FMA
-contraction.  Packed-FP32 instruction selection is disabled for all device
-code.
+coupling) are built with -ffp-contract=off; the LBM kernels with
+-ffp-contract=on and explicit fmaf, so every kernel that inlines the
+collision rounds it the same way.  Packed-FP32 instruction selection is
+disabled for all device code.
 """
 from __future__ import annotations
 
@@ -33,8 +34,13 @@ COMMON = [
 ]
 # sources whose arithmetic must be rounded operation by operation
 NO_CONTRACT = {"gel_kernels.hip", "ibm_kernels.hip"}
+# sources whose multiply-adds are fused only within one source expression:
+# the same collision code then rounds identically in every kernel it is
+# inlined into (single lattice, batched, persistent)
+CONTRACT_ON = {"lbm_kernels.hip"}
 SOURCES = ["gel_kernels.hip", "ibm_kernels.hip", "lbm_kernels.hip",
-           "runtime.cpp", "gel.cpp", "coupling.cpp", "fluid.cpp", "capi.cpp"]
+           "runtime.cpp", "gel.cpp", "coupling.cpp", "fluid.cpp", "batch.cpp", "dat_format.cpp",
+           "output.cpp", "capi.cpp"]
 
 
 def _compile(src: str) -> str:
@@ -42,6 +48,8 @@ def _compile(src: str) -> str:
     cmd = [HIPCC] + COMMON
     if src in NO_CONTRACT:
         cmd.append("-ffp-contract=off")
+    if src in CONTRACT_ON:
+        cmd.append("-ffp-contract=on")
     if src.endswith(".cpp"):
         cmd += ["-x", "hip"]
     cmd += ["-c", os.path.join(CSRC, src), "-o", obj]

@@ -1,12 +1,14 @@
 // capi.cpp -- the C ABI of include/glsm_fluid.h over the Gel / Coupler /
 // Fluid classes.  Every entry point catches C++ exceptions and HIP errors and
 // returns a status; the message is kept per thread for glsm_last_error().
+#include <cstdio>
 #include <cstring>
 #include <new>
 #include <stdexcept>
 #include <string>
 #include <vector>
 
+#include "batch.h"
 #include "coupling.h"
 #include "fluid.h"
 #include "gel.h"
@@ -22,6 +24,10 @@ namespace {
 
 thread_local std::string t_last_error;
 
+struct IoError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
 template <class F>
 int guarded(F&& f) {
   try {
@@ -30,6 +36,9 @@ int guarded(F&& f) {
   } catch (const std::invalid_argument& e) {
     t_last_error = e.what();
     return GLSM_EINVAL;
+  } catch (const IoError& e) {
+    t_last_error = e.what();
+    return GLSM_EIO;
   } catch (const std::bad_alloc& e) {
     t_last_error = e.what();
     return GLSM_ENOMEM;
@@ -147,6 +156,176 @@ void h2d(void* dev, const void* host, size_t bytes) {
   GLSM_HIP(hipStreamSynchronize(s));
 }
 
+// ---- binary state snapshots ----------------------------------------------
+// File = header, then records (id, byte count, bytes).  Gel records are the
+// device fields by their glsm_gel_field number plus the parameter block;
+// fluid records are the stored populations in their storage representation,
+// u, c and the parameter block.  A snapshot restores the state bit for bit.
+constexpr uint32_t kSnapVersion = 1;
+constexpr uint32_t kSnapGel = 1, kSnapFluid = 2;
+constexpr uint32_t kRecGelParams = 1000, kRecGelAnchor = 1001;
+constexpr uint32_t kRecFluidParams = 2000, kRecFluidRamp = 2001, kRecFluidF = 2002, kRecFluidU = 2003,
+                   kRecFluidC = 2004;
+
+struct SnapHeader {
+  char magic[8];
+  uint32_t version, kind;
+  int32_t dims[3];
+  int32_t extra[5];
+  uint64_t records;
+};
+
+struct SnapWriter {
+  FILE* fp;
+  explicit SnapWriter(const char* path) : fp(std::fopen(path, "wb")) {
+    if (!fp) throw IoError(std::string("snapshot: cannot open ") + path + " for writing");
+  }
+  ~SnapWriter() {
+    if (fp) std::fclose(fp);
+  }
+  void raw(const void* p, size_t n) {
+    if (std::fwrite(p, 1, n, fp) != n) throw IoError("snapshot: write failed");
+  }
+  void host(uint32_t id, const void* p, size_t n) {
+    const uint64_t b = n;
+    raw(&id, sizeof id);
+    raw(&b, sizeof b);
+    raw(p, n);
+  }
+  void device(uint32_t id, const void* dev, size_t n) {
+    std::vector<char> tmp(n);
+    d2h(tmp.data(), dev, n);
+    host(id, tmp.data(), n);
+  }
+};
+
+struct SnapReader {
+  FILE* fp;
+  explicit SnapReader(const char* path) : fp(std::fopen(path, "rb")) {
+    if (!fp) throw IoError(std::string("snapshot: cannot open ") + path);
+  }
+  ~SnapReader() {
+    if (fp) std::fclose(fp);
+  }
+  void raw(void* p, size_t n) {
+    if (std::fread(p, 1, n, fp) != n) throw IoError("snapshot: truncated file");
+  }
+  // next record, which must be `id` of exactly `n` bytes
+  void expect(uint32_t id, size_t n, void* dst) {
+    uint32_t rid;
+    uint64_t b;
+    raw(&rid, sizeof rid);
+    raw(&b, sizeof b);
+    if (rid != id || b != n)
+      throw std::invalid_argument("snapshot: record " + std::to_string(rid) + " (" + std::to_string(b) +
+                                  " B) where " + std::to_string(id) + " (" + std::to_string(n) + " B) was expected");
+    raw(dst, n);
+  }
+  void expect_device(uint32_t id, size_t n, void* dev) {
+    std::vector<char> tmp(n);
+    expect(id, n, tmp.data());
+    h2d(dev, tmp.data(), n);
+  }
+};
+
+SnapHeader make_header(uint32_t kind, int3 dims) {
+  SnapHeader h{};
+  std::memcpy(h.magic, "GLSMSNAP", 8);
+  h.version = kSnapVersion;
+  h.kind = kind;
+  h.dims[0] = dims.x;
+  h.dims[1] = dims.y;
+  h.dims[2] = dims.z;
+  return h;
+}
+
+void check_header(const SnapHeader& h, const SnapHeader& want, const char* what) {
+  if (std::memcmp(h.magic, "GLSMSNAP", 8) != 0 || h.version != kSnapVersion)
+    throw std::invalid_argument(std::string(what) + " snapshot: not a snapshot file of this version");
+  if (h.kind != want.kind) throw std::invalid_argument(std::string(what) + " snapshot: file holds another object kind");
+  for (int i = 0; i < 3; ++i)
+    if (h.dims[i] != want.dims[i]) throw std::invalid_argument(std::string(what) + " snapshot: grid differs");
+}
+
+void gel_save(Gel* g, const char* path) {
+  SnapHeader h = make_header(kSnapGel, g->m_gelNodeGrid);
+  h.extra[0] = g->m_gelType;
+  h.extra[1] = g->m_boundaryCount;
+  h.extra[2] = g->m_hgp->maxFilamentlen;
+  h.extra[3] = g->flag ? 1 : 0;
+  h.extra[4] = g->m_use_anchor ? 1 : 0;
+  h.records = 2 + GLSM_GEL_NFIELDS;
+  GLSM_HIP(hipStreamSynchronize(glsm::stream()));
+  SnapWriter w(path);
+  w.raw(&h, sizeof h);
+  w.host(kRecGelParams, g->m_hgp, sizeof(GelParams));
+  w.host(kRecGelAnchor, &g->m_anchor_z, sizeof(double));
+  for (int f = 0; f < GLSM_GEL_NFIELDS; ++f) {
+    const FieldRef r = gel_field(g, f);
+    w.device((uint32_t)f, r.dev, r.bytes);
+  }
+}
+
+void gel_load(Gel* g, const char* path) {
+  SnapReader rd(path);
+  SnapHeader h;
+  rd.raw(&h, sizeof h);
+  check_header(h, make_header(kSnapGel, g->m_gelNodeGrid), "gel");
+  if (h.extra[0] != g->m_gelType || h.extra[1] != g->m_boundaryCount || h.extra[2] != g->m_hgp->maxFilamentlen)
+    throw std::invalid_argument("gel snapshot: gel type or sizes differ");
+  GelParams gp;
+  rd.expect(kRecGelParams, sizeof gp, &gp);
+  double anchor;
+  rd.expect(kRecGelAnchor, sizeof anchor, &anchor);
+  for (int f = 0; f < GLSM_GEL_NFIELDS; ++f) {
+    const FieldRef r = gel_field(g, f);
+    rd.expect_device((uint32_t)f, r.bytes, r.dev);
+  }
+  *g->m_hgp = gp;
+  g->m_anchor_z = anchor;
+  g->flag = h.extra[3] != 0;
+  g->m_use_anchor = h.extra[4] != 0;
+  g->copyDataToHost();    // host mirrors follow the restored device state
+  g->copyDataToDevice();  // uploads params; marks positions as changed
+}
+
+void fluid_save(Fluid* f, const char* path) {
+  SnapHeader h = make_header(kSnapFluid, f->fluidNodeGrid);
+  h.extra[0] = f->storage();
+  h.extra[1] = f->storageScaleExp();
+  h.extra[2] = f->Nsub;
+  h.extra[3] = f->h_fp->M;
+  h.records = 5;
+  GLSM_HIP(hipStreamSynchronize(glsm::stream()));
+  SnapWriter w(path);
+  w.raw(&h, sizeof h);
+  w.host(kRecFluidParams, f->h_fp, sizeof(FluidParams));
+  w.host(kRecFluidRamp, &f->ramp_steps, sizeof(float));
+  w.device(kRecFluidF, f->d_f, f->Nd * glsm::storage_bytes(f->storage()));
+  w.device(kRecFluidU, f->d_u, f->N * sizeof(float3));
+  w.device(kRecFluidC, f->d_c1, f->N * sizeof(float));
+}
+
+void fluid_load(Fluid* f, const char* path) {
+  SnapReader rd(path);
+  SnapHeader h;
+  rd.raw(&h, sizeof h);
+  check_header(h, make_header(kSnapFluid, f->fluidNodeGrid), "fluid");
+  if (h.extra[2] != f->Nsub || h.extra[3] != f->h_fp->M)
+    throw std::invalid_argument("fluid snapshot: sub-step or marker count differs");
+  FluidParams fp;
+  rd.expect(kRecFluidParams, sizeof fp, &fp);
+  float ramp;
+  rd.expect(kRecFluidRamp, sizeof ramp, &ramp);
+  if (h.extra[0] != f->storage() || h.extra[1] != f->storageScaleExp()) f->setStorage(h.extra[0], h.extra[1]);
+  rd.expect_device(kRecFluidF, f->Nd * glsm::storage_bytes(f->storage()), f->d_f);
+  rd.expect_device(kRecFluidU, f->N * sizeof(float3), f->d_u);
+  rd.expect_device(kRecFluidC, f->N * sizeof(float), f->d_c1);
+  *f->h_fp = fp;
+  f->ramp_steps = ramp;
+  f->setPeriodic(fp.perX, fp.perY, fp.perZ);  // also forces the IBM lists to be rebuilt
+}
+
 }  // namespace
 
 extern "C" {
@@ -199,6 +378,20 @@ int glsm_gel_record_center(glsm_gel* g, int iter) {
 int glsm_gel_write_files(glsm_gel* g, int iter) {
   return guarded([&] { G(g)->writeFiles(iter); return GLSM_OK; });
 }
+int glsm_gel_save_state(glsm_gel* g, const char* path) {
+  return guarded([&] {
+    REQUIRE(path, "null path");
+    gel_save(G(g), path);
+    return GLSM_OK;
+  });
+}
+int glsm_gel_load_state(glsm_gel* g, const char* path) {
+  return guarded([&] {
+    REQUIRE(path, "null path");
+    gel_load(G(g), path);
+    return GLSM_OK;
+  });
+}
 int glsm_gel_reset_to_quiescent(glsm_gel* g) {
   return guarded([&] { G(g)->resetToQuiescent(); return GLSM_OK; });
 }
@@ -228,6 +421,9 @@ int glsm_gel_dims(glsm_gel* g, int grid[3], int* nN, int* nE, int* M) {
     *M = p->m_boundaryCount;
     return GLSM_OK;
   });
+}
+int glsm_gel_set_chemistry_rates(glsm_gel* g, double f, double ep) {
+  return guarded([&] { G(g)->setChemistryRates(f, ep); return GLSM_OK; });
 }
 int glsm_gel_perturb_chemistry(glsm_gel* g, uint64_t seed, double amplitude) {
   return guarded([&] { G(g)->perturbChemistry(seed, amplitude); return GLSM_OK; });
@@ -338,6 +534,27 @@ int glsm_fluid_step_concentration(glsm_fluid* f, int iter) {
 int glsm_fluid_write_files(glsm_fluid* f, int iter) {
   return guarded([&] { F(f)->writeFiles(iter); return GLSM_OK; });
 }
+int glsm_fluid_save_state(glsm_fluid* f, const char* path) {
+  return guarded([&] {
+    REQUIRE(path, "null path");
+    fluid_save(F(f), path);
+    return GLSM_OK;
+  });
+}
+int glsm_fluid_load_state(glsm_fluid* f, const char* path) {
+  return guarded([&] {
+    REQUIRE(path, "null path");
+    fluid_load(F(f), path);
+    return GLSM_OK;
+  });
+}
+int glsm_fluid_conservation(glsm_fluid* f, double out[4]) {
+  return guarded([&] {
+    REQUIRE(out, "null output pointer");
+    F(f)->conservation(out);
+    return GLSM_OK;
+  });
+}
 int glsm_fluid_set_periodic(glsm_fluid* f, int px, int py, int pz) {
   return guarded([&] { F(f)->setPeriodic(px, py, pz); return GLSM_OK; });
 }
@@ -423,6 +640,103 @@ int glsm_copy_bandwidth(size_t bytes, int reps, double* gbps) {
     glsm::dfree(a);
     glsm::dfree(b);
     *gbps = 2.0 * (double)bytes * reps / (ms * 1e-3) / 1e9;
+    return GLSM_OK;
+  });
+}
+
+// ---------------------------------------------------------------- batch
+struct glsm_batch {
+  Batch* b = nullptr;
+};
+namespace {
+Batch* B(glsm_batch* b) {
+  REQUIRE(b && b->b, "null batch handle");
+  return b->b;
+}
+}  // namespace
+int glsm_batch_create(int storage, int scale_exp, glsm_batch** out) {
+  return guarded([&] {
+    REQUIRE(out, "null output pointer");
+    auto* h = new glsm_batch;
+    try {
+      h->b = new Batch(storage, scale_exp);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+    return GLSM_OK;
+  });
+}
+int glsm_batch_destroy(glsm_batch* b) {
+  return guarded([&] {
+    delete B(b);
+    delete b;
+    return GLSM_OK;
+  });
+}
+int glsm_batch_add(glsm_batch* b, glsm_coupler* c, glsm_fluid* f, int iter, int write_files) {
+  return guarded([&] {
+    glsm_coupler* cp = CP(c);
+    return B(b)->add(cp->gels, cp->c, F(f), iter, write_files != 0);
+  });
+}
+int glsm_batch_remove(glsm_batch* b, int id) {
+  return guarded([&] {
+    B(b)->remove(id);
+    return GLSM_OK;
+  });
+}
+int glsm_batch_size(glsm_batch* b) {
+  return guarded([&] { return B(b)->size(); });
+}
+int glsm_batch_iteration(glsm_batch* b, int id) {
+  return guarded([&] { return B(b)->iteration(id); });
+}
+int glsm_batch_step(glsm_batch* b, int nsteps) {
+  return guarded([&] {
+    B(b)->step(nsteps);
+    return GLSM_OK;
+  });
+}
+int glsm_batch_use_graph(glsm_batch* b, int on) {
+  return guarded([&] {
+    B(b)->useGraph(on != 0);
+    return GLSM_OK;
+  });
+}
+int glsm_batch_set_launch(glsm_batch* b, int nt, int run, int grid) {
+  return guarded([&] {
+    B(b)->setLaunch(Batch::Launch{nt, run, grid});
+    return GLSM_OK;
+  });
+}
+int glsm_batch_get_launch(glsm_batch* b, int* nt, int* run, int* grid) {
+  return guarded([&] {
+    REQUIRE(nt && run && grid, "null output pointer");
+    const Batch::Launch l = B(b)->launch();
+    *nt = l.nt;
+    *run = l.run;
+    *grid = l.grid;
+    return GLSM_OK;
+  });
+}
+int glsm_batch_autotune(glsm_batch* b, int reps, double* best_ms) {
+  return guarded([&] {
+    B(b)->autotune(reps, best_ms);
+    return GLSM_OK;
+  });
+}
+int glsm_batch_enable_lbm_timing(glsm_batch* b, int every) {
+  return guarded([&] {
+    B(b)->enableLbmTiming(every);
+    return GLSM_OK;
+  });
+}
+int glsm_batch_lbm_timing(glsm_batch* b, double* mean_ms, int* samples) {
+  return guarded([&] {
+    REQUIRE(mean_ms && samples, "null output pointer");
+    *mean_ms = B(b)->lbmKernelTiming(samples);
     return GLSM_OK;
   });
 }

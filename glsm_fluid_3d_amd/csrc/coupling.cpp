@@ -122,4 +122,5 @@ void Coupler::applyGelRepulsion() {
 namespace glsm {
 // marker arrays for the lattice-side kernels (which never touch the gels)
 MarkerArrays markers_of(Coupler& c) { return marker_arrays(c, nullptr); }
+MarkerArrays markers_with_gels(Coupler& c) { return marker_arrays(c, c.deviceGelTable()); }
 }  // namespace glsm

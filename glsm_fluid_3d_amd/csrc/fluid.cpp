@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
@@ -56,6 +57,11 @@ int storage_from_env(int dflt) {
 }
 
 }  // namespace
+
+namespace glsm {
+LatticeGeom fluid_geom(const Fluid& f) { return geom(f); }
+LbmParams fluid_lbm_params(const Fluid& f) { return lbm_params(*f.h_fp, f.storage(), f.storageScaleExp()); }
+}  // namespace glsm
 
 int Fluid::idx3(int x, int y, int z, int Nx, int Ny) { return x + y * Nx + z * Nx * Ny; }
 
@@ -146,6 +152,7 @@ Fluid::Fluid(int3 size, int time, Coupler* cp)
   d_fillpos_ = glsm::dalloc<int>(N);
   d_rng_ = glsm::dalloc<int2>(N);
   d_bsum_ = glsm::dalloc<int>(nblk_ + 1);
+  d_red_ = glsm::dalloc<double>(4ull * nblk_ + 4);
   ent_cap_ = 64 * std::max(1, coupler->sumGelBoundaryCount);
   d_ent_ = glsm::dalloc<SpreadEntry>(ent_cap_);
   d_ctab_ = glsm::dalloc<int>(ent_cap_);
@@ -167,7 +174,7 @@ Fluid::~Fluid() {
   (void)hipHostFree(h_c1);
   delete h_fp;
   for (void* p : {(void*)d_f, (void*)d_fnext, (void*)d_u, (void*)d_c1, (void*)d_c2, (void*)d_blkflag_, (void*)d_cnt_,
-                  (void*)d_fillpos_, (void*)d_rng_, (void*)d_bsum_, (void*)d_ent_, (void*)d_ctab_, (void*)d_wtab_,
+                  (void*)d_fillpos_, (void*)d_rng_, (void*)d_bsum_, (void*)d_red_, (void*)d_ent_, (void*)d_ctab_, (void*)d_wtab_,
                   (void*)d_rtab_, (void*)d_wsum_})
     glsm::dfree(p);
 }
@@ -323,16 +330,7 @@ void Fluid::copyDataToHost() {
 
 // fluid.cu:115-138
 void Fluid::recordData(int time) {
-  ofstream fVelb("Velb" + to_string(time) + ".dat");
-  ofstream fC1("Conc" + to_string(time) + ".dat");
-  for (int zi = 0; zi < fluidNodeGrid.z; zi++)
-    for (int yi = 0; yi < fluidNodeGrid.y; yi++)
-      for (int xi = 0; xi < fluidNodeGrid.x; xi++) {
-        const int gi = idx3(xi, yi, zi, fluidNodeGrid.x, fluidNodeGrid.y);
-        fVelb << setw(9) << to_string(h_u[gi].x) << "      " << setw(9) << to_string(h_u[gi].y) << "      "
-              << setw(9) << to_string(h_u[gi].z) << "\n";
-        fC1 << setw(9) << to_string(h_c1[gi]) << "\n";
-      }
+  glsm::write_fluid_dat("", time, fluidNodeGrid.x, fluidNodeGrid.y, fluidNodeGrid.z, h_u, h_c1);
 }
 
 // fluid.cu:140-150 (iter * dt is a float product there)
@@ -341,8 +339,23 @@ void Fluid::writeFiles(int iter) {
   if (iter % 2000 == 0) {
     if (file_writer_thread.joinable()) file_writer_thread.join();
     copyDataToHost();
-    file_writer_thread = std::thread(&Fluid::recordData, this, int(time));
+    // a failed write is reported, not thrown: the reference's ofstreams fail silently
+    file_writer_thread = std::thread([this, t = int(time)] {
+      try {
+        recordData(t);
+      } catch (const std::exception& e) {
+        std::fprintf(stderr, "fluid: %s\n", e.what());
+      }
+    });
   }
+}
+
+void Fluid::conservation(double out[4]) {
+  double* dout = d_red_ + 4ull * nblk_;
+  glsm::launch_lbm_moments(storage_, geom(*this), lbm_params(*h_fp, storage_, scale_exp_), d_f, plane, d_red_, dout,
+                           fluid_stream);
+  GLSM_HIP(hipMemcpyAsync(out, dout, 4 * sizeof(double), hipMemcpyDeviceToHost, fluid_stream));
+  GLSM_HIP(hipStreamSynchronize(fluid_stream));
 }
 
 void Fluid::readPopulations(float* host) {

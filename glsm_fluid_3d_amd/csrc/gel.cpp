@@ -3,11 +3,13 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <fstream>
 #include <iomanip>
 #include <iostream>
 #include <sstream>
+#include <stdexcept>
 
 #include "glsm_internal.hpp"
 
@@ -69,6 +71,10 @@ void d2h(T* h, const T* d, size_t n, hipStream_t s) {
 }
 
 }  // namespace
+
+namespace glsm {
+GelView gel_view(Gel& g) { return make_view(g); }
+}  // namespace glsm
 
 // gel.cu:555-663
 Gel::Gel(int3 gelSize, double3 gelPosition, int gelType, int gelId, int time)
@@ -401,34 +407,9 @@ void Gel::recordCenter(int iter) {
 
 // gel.cu:452-541: text snapshot of one output time
 void Gel::recordData(int time) {
-  const int LX = m_gelNodeGrid.x, LY = m_gelNodeGrid.y, LZ = m_gelNodeGrid.z;
-  const string id = to_string(m_gelId), t = to_string(time);
-  {
-    ofstream frn("gel" + id + "rn" + t + ".dat"), frm("gel" + id + "rm" + t + ".dat"),
-        fum("gel" + id + "um" + t + ".dat"), fvm("gel" + id + "vm" + t + ".dat"), fwm("gel" + id + "wm" + t + ".dat"),
-        fFn("gel" + id + "Fn" + t + ".dat"), fVeln("gel" + id + "Veln" + t + ".dat");
-    auto row3 = [](ofstream& o, const double3& a) {
-      o << setw(9) << to_string(a.x) << "      " << setw(9) << to_string(a.y) << "      " << setw(9)
-        << to_string(a.z) << "\n";
-    };
-    for (int zi = 1; zi < LZ + 1; zi++)
-      for (int yi = 1; yi < LY + 1; yi++)
-        for (int xi = 1; xi < LX + 1; xi++) {
-          int gi = get_index(xi, yi, zi, 2);
-          row3(frn, m_hrn[gi]);
-          row3(fFn, m_hFn[gi]);
-          row3(fVeln, m_hVeln[gi]);
-        }
-    for (int zi = 1; zi < LZ; zi++)
-      for (int yi = 1; yi < LY; yi++)
-        for (int xi = 1; xi < LX; xi++) {
-          int gi = get_index(xi, yi, zi, 1);
-          row3(frm, m_hrm[gi]);
-          fum << setw(9) << to_string(m_hum[gi]) << "\n";
-          fvm << setw(9) << m_hvm[gi] << "\n";
-          fwm << setw(9) << m_hwm[gi] << "\n";
-        }
-  }
+  const string id = to_string(m_gelId);
+  glsm::write_gel_dat("", m_gelId, time, m_gelNodeGrid.x, m_gelNodeGrid.y, m_gelNodeGrid.z, m_hrn, m_hFn, m_hVeln,
+                      m_hrm, m_hum, m_hvm, m_hwm);
   if (flag && time % 1000 == 0) {
     ofstream fb("gel" + id + "bodycenter" + ".dat", ios::app);
     for (int i = 0; i < 1000; i++) {
@@ -451,11 +432,35 @@ void Gel::writeFiles(int iter) {
   if (iter % 1000 == 0) {
     if (m_file_writer_thread.joinable()) m_file_writer_thread.join();
     copyDataToHost();
-    m_file_writer_thread = std::thread(&Gel::recordData, this, int(time));
+    // a failed write is reported, not thrown: the reference's ofstreams fail silently
+    m_file_writer_thread = std::thread([this, t = int(time)] {
+      try {
+        recordData(t);
+      } catch (const std::exception& e) {
+        std::fprintf(stderr, "gel %d: %s\n", m_gelId, e.what());
+      }
+    });
   }
 }
 
 // gel.cu:731-747
+void Gel::setChemistryRates(double f, double ep) {
+  if (!(f > 0.0) || !(ep > 0.0)) throw std::invalid_argument("setChemistryRates: f and ep must be positive");
+  m_hgp->f = f;
+  m_hgp->ep = ep;
+  steadyStateValue(m_hgp->uss, m_hgp->vss, m_hgp->wss, 0);
+  for (int zi = 1; zi < m_gelNodeGrid.z; zi++)
+    for (int yi = 1; yi < m_gelNodeGrid.y; yi++)
+      for (int xi = 1; xi < m_gelNodeGrid.x; xi++) {
+        const int gi = get_index(xi, yi, zi, 1);
+        m_hum[gi] = m_hgp->uss;
+        m_hvm[gi] = m_hgp->vss;
+        m_hwm[gi] = m_hgp->wss;
+      }
+  GLSM_HIP(hipMemcpyAsync(m_dgp, m_hgp, sizeof(GelParams), hipMemcpyHostToDevice, m_gel_stream));
+  uploadChemistry(true);
+}
+
 void Gel::resetToQuiescent() {
   const int LX = m_gelNodeGrid.x, LY = m_gelNodeGrid.y, LZ = m_gelNodeGrid.z;
   for (int zi = 1; zi <= LZ; zi++)

@@ -44,9 +44,9 @@ struct Range3 {  // inclusive box [lo, hi]^3 (per axis), flattened x fastest
     z = z0 + r / ey;
     return true;
   }
-  int count() const { return ex * ey * ez; }
+  __host__ __device__ int count() const { return ex * ey * ez; }
 };
-inline Range3 box(int lo, int hx, int hy, int hz) {
+__host__ __device__ inline Range3 box(int lo, int hx, int hy, int hz) {
   Range3 r{lo, lo, lo, hx - lo + 1, hy - lo + 1, hz - lo + 1};
   if (r.ex < 0) r.ex = 0;
   if (r.ey < 0) r.ey = 0;
@@ -64,9 +64,7 @@ __device__ inline int nodeIdx(const GelView& g, int x, int y, int z) {
 
 // ghost node = 2*(first real node) - (second real node) along the type's
 // offset (calServiceNodesPositionD, gel_kernels.cu:615-631)
-__global__ void __launch_bounds__(kThreads) k_service_nodes(GelView g, Range3 r) {
-  int x, y, z;
-  if (!r.decode(blockIdx.x * blockDim.x + threadIdx.x, x, y, z)) return;
+__device__ __forceinline__ void service_nodes_at(const GelView& g, int x, int y, int z) {
   const int gi = nodeIdx(g, x, y, z);
   const int t = g.map_node[gi];
   if (t == 0) return;
@@ -91,9 +89,7 @@ __device__ inline double dshape(int i, int axis, double p0, double p1, double p2
 // centroid, local node offsets, six face area vectors (2x2 midpoint rule of
 // the Jacobian cross products), volume (2x2x2 rule of det J), wm update
 // (calElementPropertiesD, gel_kernels.cu:216-483)
-__global__ void __launch_bounds__(kThreads) k_element_props(GelView g, Range3 r) {
-  int x, y, z;
-  if (!r.decode(blockIdx.x * blockDim.x + threadIdx.x, x, y, z)) return;
+__device__ __forceinline__ void element_props_at(const GelView& g, int x, int y, int z) {
   const int gi = elemIdx(g, x, y, z);
   double3 nd[8];
   double3 bc = d3(0, 0, 0);
@@ -169,9 +165,7 @@ __global__ void __launch_bounds__(kThreads) k_element_props(GelView g, Range3 r)
 // every element (calChemBoundaryD, gel_kernels.cu:757-778); interior
 // elements also get their osmotic pressure (calPressureD,
 // gel_kernels.cu:496-516), which only reads the element's own w and v.
-__global__ void __launch_bounds__(kThreads) k_chem_boundary_pressure(GelView g, Range3 r) {
-  int x, y, z;
-  if (!r.decode(blockIdx.x * blockDim.x + threadIdx.x, x, y, z)) return;
+__device__ __forceinline__ void chem_boundary_pressure_at(const GelView& g, int x, int y, int z) {
   const int gi = elemIdx(g, x, y, z);
   const int t = g.map_element[gi];
   const int g1 = elemIdx(g, x + g.off[t][0], y + g.off[t][1], z + g.off[t][2]);
@@ -194,9 +188,7 @@ __global__ void __launch_bounds__(kThreads) k_chem_boundary_pressure(GelView g, 
 // node averages of the normalised chemistry (calUnnormD,
 // gel_kernels.cu:795-821) then the node force, mobility and velocities
 // (calNodesVelocityD, gel_kernels.cu:518-598).  Both are per real node.
-__global__ void __launch_bounds__(kThreads) k_unnorm_velocity(GelView g, Range3 r) {
-  int x, y, z;
-  if (!r.decode(blockIdx.x * blockDim.x + threadIdx.x, x, y, z)) return;
+__device__ __forceinline__ void unnorm_velocity_at(const GelView& g, int x, int y, int z) {
   const int gi = nodeIdx(g, x, y, z);
   double su = 0, sv = 0, wn = 0;
 #pragma unroll
@@ -256,9 +248,7 @@ __global__ void __launch_bounds__(kThreads) k_unnorm_velocity(GelView g, Range3 
 // the optional bottom anchor (anchorBottomNodesD, gel_kernels.cu:870-880).
 // The reference launches the anchor on gridDim0 with no +1 shift
 // (gel.cu:694), so it reaches x, y only below 8*ceil(L/8); kept here.
-__global__ void __launch_bounds__(kThreads) k_position_anchor(GelView g, Range3 r) {
-  int x, y, z;
-  if (!r.decode(blockIdx.x * blockDim.x + threadIdx.x, x, y, z)) return;
+__device__ __forceinline__ void position_anchor_at(const GelView& g, int x, int y, int z) {
   const int gi = nodeIdx(g, x, y, z);
   double3 p = g.rn[gi];
   acc3(p, scal(g.dtx, g.Veln[gi]));
@@ -272,9 +262,7 @@ __global__ void __launch_bounds__(kThreads) k_position_anchor(GelView g, Range3 
 // chemistry source terms T0 (dilation), T1 (advective flux through the six
 // faces), T2 (non-uniform 3-point gradient/Laplacian) (calTermsD,
 // gel_kernels.cu:633-703)
-__global__ void __launch_bounds__(kThreads) k_terms(GelView g, Range3 r) {
-  int x, y, z;
-  if (!r.decode(blockIdx.x * blockDim.x + threadIdx.x, x, y, z)) return;
+__device__ __forceinline__ void terms_at(const GelView& g, int x, int y, int z) {
   const int gi = elemIdx(g, x, y, z);
   const double w0 = g.wm[gi], u0 = g.um_norm[gi];
   const double3 r0 = g.rm[gi];
@@ -310,9 +298,7 @@ __global__ void __launch_bounds__(kThreads) k_terms(GelView g, Range3 r) {
   g.T2[gi] = -dot3(dw, du) + (1 - w0) * (d2u.x + d2u.y + d2u.z);
 }
 
-__global__ void __launch_bounds__(kThreads) k_set_zero(GelView g, Range3 r) {
-  int x, y, z;
-  if (!r.decode(blockIdx.x * blockDim.x + threadIdx.x, x, y, z)) return;
+__device__ __forceinline__ void set_zero_at(const GelView& g, int x, int y, int z) {
   const int gi = nodeIdx(g, x, y, z);
   g.un_robin[gi] = 0;
   g.Fn_robin[gi] = d3(0., 0., 0.);
@@ -330,9 +316,7 @@ __device__ inline bool void_element(const GelView& g, int y, int z) {
 
 // forward-Euler modified Oregonator (calChemD + fu/fv, gel_kernels.cu:705-755,
 // 129-145; q, P1, P2 are the kernels' literals)
-__global__ void __launch_bounds__(kThreads) k_chem(GelView g, Range3 r) {
-  int x, y, z;
-  if (!r.decode(blockIdx.x * blockDim.x + threadIdx.x, x, y, z)) return;
+__device__ __forceinline__ void chem_at(const GelView& g, int x, int y, int z) {
   const int gi = elemIdx(g, x, y, z);
   if (void_element(g, y, z)) {
     g.vm[gi] = g.vss;
@@ -391,7 +375,7 @@ __device__ bool solve_bilinear(double A, double B, double C, double D, double E,
 // gel_kernels.cu:837-866).  One work-group; hits are compacted in element
 // order with a block prefix sum (the reference's atomic counter gives an
 // arbitrary order) and bounded by maxFilamentlen.
-__global__ void __launch_bounds__(kThreads) k_record_center(GelView g, int slot) {
+__device__ __forceinline__ void record_center_block(const GelView& g, int slot) {
   __shared__ int s_scan[kThreads];
   __shared__ int s_base;
   if (threadIdx.x == 0) {
@@ -438,37 +422,121 @@ __global__ void __launch_bounds__(kThreads) k_record_center(GelView g, int slot)
   }
 }
 
+// ---- launch shapes ----
+// Each phase runs over one box of a gel's node or element array; the
+// single-gel kernel takes the box, the batched kernel derives it per gel.
+enum GelOp { OP_SERVICE, OP_PROPS, OP_CHEMB, OP_UNNORM, OP_POS, OP_TERMS, OP_SETZERO, OP_CHEM };
+
+template <int OP>
+__device__ __forceinline__ void gel_op(const GelView& g, int x, int y, int z) {
+  if constexpr (OP == OP_SERVICE) service_nodes_at(g, x, y, z);
+  if constexpr (OP == OP_PROPS) element_props_at(g, x, y, z);
+  if constexpr (OP == OP_CHEMB) chem_boundary_pressure_at(g, x, y, z);
+  if constexpr (OP == OP_UNNORM) unnorm_velocity_at(g, x, y, z);
+  if constexpr (OP == OP_POS) position_anchor_at(g, x, y, z);
+  if constexpr (OP == OP_TERMS) terms_at(g, x, y, z);
+  if constexpr (OP == OP_SETZERO) set_zero_at(g, x, y, z);
+  if constexpr (OP == OP_CHEM) chem_at(g, x, y, z);
+}
+
+// nodes_all, elements_all, real nodes, interior elements
+template <int OP>
+__host__ __device__ inline Range3 op_range(int LX, int LY, int LZ) {
+  if (OP == OP_SERVICE) return box(0, LX + 1, LY + 1, LZ + 1);
+  if (OP == OP_PROPS || OP == OP_CHEMB) return box(0, LX, LY, LZ);
+  if (OP == OP_UNNORM || OP == OP_POS || OP == OP_SETZERO) return box(1, LX, LY, LZ);
+  return box(1, LX - 1, LY - 1, LZ - 1);
+}
+
+template <int OP>
+__global__ void __launch_bounds__(kThreads) k_gel(GelView g, Range3 r) {
+  int x, y, z;
+  if (!r.decode(blockIdx.x * blockDim.x + threadIdx.x, x, y, z)) return;
+  gel_op<OP>(g, x, y, z);
+}
+
+// All gels of a batch in one launch: blockIdx.y selects the gel; a gel
+// takes part when its instance's iteration is a multiple of `cadence`.
+template <int OP>
+__global__ void __launch_bounds__(kThreads) k_gel_batch(const GelView* __restrict__ views,
+                                                        const int* __restrict__ gel_inst,
+                                                        const int* __restrict__ iters, int cadence) {
+  const GelView& g = views[blockIdx.y];
+  if (cadence > 1 && iters[gel_inst[blockIdx.y]] % cadence != 0) return;
+  const Range3 r = op_range<OP>(g.LX, g.LY, g.LZ);
+  int x, y, z;
+  if (!r.decode(blockIdx.x * blockDim.x + threadIdx.x, x, y, z)) return;
+  gel_op<OP>(g, x, y, z);
+}
+
+__global__ void __launch_bounds__(kThreads) k_record_center(GelView g, int slot) { record_center_block(g, slot); }
+
+// every 1000th iteration of the gel's instance: slot (iter / 1000) % 1000
+__global__ void __launch_bounds__(kThreads) k_record_center_batch(const GelView* __restrict__ views,
+                                                                  const int* __restrict__ gel_inst,
+                                                                  const int* __restrict__ iters) {
+  const int it = iters[gel_inst[blockIdx.x]];
+  if (it % 1000 != 0) return;
+  record_center_block(views[blockIdx.x], (it / 1000) % 1000);
+}
+
+template <int OP>
+void launch_op(const GelView& g, hipStream_t s) {
+  const Range3 r = op_range<OP>(g.LX, g.LY, g.LZ);
+  if (r.count() > 0) k_gel<OP><<<blocks_for(r.count()), kThreads, 0, s>>>(g, r);
+}
+
+template <int OP>
+void launch_op_batch(const GelView* views, const int* gel_inst, const int* iters, int ngel, int max_count,
+                     int cadence, hipStream_t s) {
+  if (ngel > 0 && max_count > 0)
+    k_gel_batch<OP><<<dim3(blocks_for(max_count), ngel), kThreads, 0, s>>>(views, gel_inst, iters, cadence);
+}
+
 }  // namespace
 
 void launch_gel_elasticity(const GelView& g, hipStream_t s) {
-  const Range3 nodes_all = box(0, g.LX + 1, g.LY + 1, g.LZ + 1);
-  const Range3 elems_all = box(0, g.LX, g.LY, g.LZ);
-  const Range3 nodes_real = box(1, g.LX, g.LY, g.LZ);
-  const Range3 elems_int = box(1, g.LX - 1, g.LY - 1, g.LZ - 1);
-  k_service_nodes<<<blocks_for(nodes_all.count()), kThreads, 0, s>>>(g, nodes_all);
-  k_element_props<<<blocks_for(elems_all.count()), kThreads, 0, s>>>(g, elems_all);
-  k_chem_boundary_pressure<<<blocks_for(elems_all.count()), kThreads, 0, s>>>(g, elems_all);
-  k_unnorm_velocity<<<blocks_for(nodes_real.count()), kThreads, 0, s>>>(g, nodes_real);
-  k_position_anchor<<<blocks_for(nodes_real.count()), kThreads, 0, s>>>(g, nodes_real);
-  if (elems_int.count() > 0) k_terms<<<blocks_for(elems_int.count()), kThreads, 0, s>>>(g, elems_int);
+  launch_op<OP_SERVICE>(g, s);
+  launch_op<OP_PROPS>(g, s);
+  launch_op<OP_CHEMB>(g, s);
+  launch_op<OP_UNNORM>(g, s);
+  launch_op<OP_POS>(g, s);
+  launch_op<OP_TERMS>(g, s);
   GLSM_LAUNCH_CHECK();
 }
 
 void launch_gel_set_zero(const GelView& g, hipStream_t s) {
-  const Range3 nodes_real = box(1, g.LX, g.LY, g.LZ);
-  k_set_zero<<<blocks_for(nodes_real.count()), kThreads, 0, s>>>(g, nodes_real);
+  launch_op<OP_SETZERO>(g, s);
   GLSM_LAUNCH_CHECK();
 }
 
 void launch_gel_chemistry(const GelView& g, hipStream_t s) {
-  const Range3 elems_int = box(1, g.LX - 1, g.LY - 1, g.LZ - 1);
-  if (elems_int.count() == 0) return;
-  k_chem<<<blocks_for(elems_int.count()), kThreads, 0, s>>>(g, elems_int);
+  launch_op<OP_CHEM>(g, s);
   GLSM_LAUNCH_CHECK();
 }
 
 void launch_gel_record_center(const GelView& g, int slot, hipStream_t s) {
   k_record_center<<<1, kThreads, 0, s>>>(g, slot);
+  GLSM_LAUNCH_CHECK();
+}
+
+// gel.cu:682-713 for every gel of a batch: elasticity on iterations that are
+// multiples of 5, setZero and chemistry always, the centre record on
+// multiples of 1000.  `max_node_grid` is the largest L over the gels.
+void launch_gel_step_batch(const GelView* views, const int* gel_inst, const int* iters, int ngel,
+                           int max_node_grid, hipStream_t s) {
+  const int L = max_node_grid;
+  const int nodes_all = (L + 2) * (L + 2) * (L + 2), elems_all = (L + 1) * (L + 1) * (L + 1);
+  const int nodes_real = L * L * L, elems_int = (L - 1) * (L - 1) * (L - 1);
+  launch_op_batch<OP_SERVICE>(views, gel_inst, iters, ngel, nodes_all, 5, s);
+  launch_op_batch<OP_PROPS>(views, gel_inst, iters, ngel, elems_all, 5, s);
+  launch_op_batch<OP_CHEMB>(views, gel_inst, iters, ngel, elems_all, 5, s);
+  launch_op_batch<OP_UNNORM>(views, gel_inst, iters, ngel, nodes_real, 5, s);
+  launch_op_batch<OP_POS>(views, gel_inst, iters, ngel, nodes_real, 5, s);
+  launch_op_batch<OP_TERMS>(views, gel_inst, iters, ngel, elems_int, 5, s);
+  launch_op_batch<OP_SETZERO>(views, gel_inst, iters, ngel, nodes_real, 1, s);
+  launch_op_batch<OP_CHEM>(views, gel_inst, iters, ngel, elems_int, 1, s);
+  if (ngel > 0) k_record_center_batch<<<ngel, kThreads, 0, s>>>(views, gel_inst, iters);
   GLSM_LAUNCH_CHECK();
 }
 

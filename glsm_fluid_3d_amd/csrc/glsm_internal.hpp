@@ -63,6 +63,11 @@ void launch_gel_elasticity(const GelView& g, hipStream_t s);  // iter % 5 == 0 b
 void launch_gel_set_zero(const GelView& g, hipStream_t s);
 void launch_gel_chemistry(const GelView& g, hipStream_t s);
 void launch_gel_record_center(const GelView& g, int slot, hipStream_t s);
+// one gel step (elasticity on multiples of 5, setZero, chemistry, centre
+// record on multiples of 1000) for `ngel` gels; views / gel_inst in device
+// memory, iters[gel_inst[g]] = the iteration of gel g's instance
+void launch_gel_step_batch(const GelView* views, const int* gel_inst, const int* iters, int ngel,
+                           int max_node_grid, hipStream_t s);
 
 // ---- coupler ----------------------------------------------------------------
 }  // namespace glsm
@@ -99,6 +104,7 @@ struct MarkerArrays {
 class Coupler;
 namespace glsm {
 MarkerArrays markers_of(Coupler& c);  // lattice-side view (no gel table)
+MarkerArrays markers_with_gels(Coupler& c);  // with the gel table (pack / scatter)
 
 void launch_pack(const MarkerArrays& m, float gel_to_lbm_vel, hipStream_t s);
 void launch_scatter(const MarkerArrays& m, hipStream_t s);
@@ -159,6 +165,10 @@ void launch_lbm_init(int storage, const LatticeGeom& g, const LbmParams& p, void
 // sub-step, else the stored post-collision ones), and the inverse of pull
 void launch_lbm_export(int storage, bool pull, const LatticeGeom& g, const LbmParams& p, const void* fpost,
                        float* out, size_t plane, size_t out_plane, hipStream_t s);
+// FP64 sum(rho - 1) and sum(rho u) of the pre-collision populations into
+// out4[0..3]; `partial` holds 4 doubles per 256-cell block
+void launch_lbm_moments(int storage, const LatticeGeom& g, const LbmParams& p, const void* fpost, size_t plane,
+                        double* partial, double* out4, hipStream_t s);
 void launch_lbm_import(int storage, const LatticeGeom& g, const LbmParams& p, const float* in, void* fpost,
                        size_t plane, size_t in_plane, hipStream_t s);
 
@@ -167,5 +177,73 @@ struct ScalarParams {
 };
 void launch_scalar(const LatticeGeom& g, const ScalarParams& p, const float3* u, const float* c1,
                    float* c2, const SpreadLists& sl, const float* Sl, hipStream_t s);
+
+// ---- .dat output (reference text layout, from host arrays) -----------------
+void write_fluid_dat(const std::string& dir, int time, int Nx, int Ny, int Nz, const float3* u, const float* c);
+void write_gel_dat(const std::string& dir, int gel_id, int time, int LX, int LY, int LZ, const double3* rn,
+                   const double3* Fn, const double3* Veln, const double3* rm, const double* um, const double* vm,
+                   const double* wm);
+
+// ---- batches --------------------------------------------------------------
+}  // namespace glsm
+class Gel;
+class Fluid;
+namespace glsm {
+GelView gel_view(Gel& g);
+LatticeGeom fluid_geom(const Fluid& f);
+LbmParams fluid_lbm_params(const Fluid& f);
+
+// One member (gel-fluid instance) of a batch as the batched kernels see it.
+// Its 256-cell blocks are [blk0, blk0 + sl.nblk) and its markers
+// [mk0, mk0 + m.M) in the batch-wide numbering.
+struct BatchInst {
+  LatticeGeom g;
+  LbmParams p;
+  ScalarParams sp;
+  size_t plane;
+  void* f[2];     // the two post-collision buffers (storage representation)
+  float3* u;
+  float* c[2];    // concentration ping-pong
+  SpreadLists sl;
+  MarkerArrays m;
+  float g2l;      // gel -> lattice velocity factor of the pack
+  float beta, ramp;
+  float delta, gamma;
+  int const_force;
+  int blk0, mk0;
+};
+
+// Device tables of a batch: members, block -> member, marker -> member and
+// the members' iteration counters.
+struct BatchTables {
+  const BatchInst* inst;
+  const int* blk_inst;
+  const int* mk_inst;
+  const int* iters;
+  int nblk, M, ninst;
+};
+
+void launch_pack_batch(const BatchTables& t, hipStream_t s);
+void launch_scatter_batch(const BatchTables& t, hipStream_t s);
+void launch_repulsion_batch(const BatchTables& t, hipStream_t s);
+void launch_ibm_interp_batch(const BatchTables& t, hipStream_t s);  // beta_eff from iters
+void launch_ibm_sample_batch(const BatchTables& t, int cpar, hipStream_t s);
+void launch_spread_rebuild_batch(const BatchTables& t, hipStream_t s);
+void launch_scalar_batch(const BatchTables& t, int cpar, hipStream_t s);
+
+// Launch shape of the batched stream/collide kernel (the autotuner's knobs):
+// non-temporal stores, XCD run length (0 = one contiguous run per XCD) and
+// the number of work-groups (0 = one per block; fewer = persistent
+// work-groups looping over the blocks).
+struct LbmLaunch {
+  int nt = 0;
+  int run = 0;
+  int grid = 0;
+};
+// one sub-step of every member: f[sel] -> f[sel ^ 1]
+void launch_lbm_step_batch(int storage, const BatchTables& t, int sel, bool write_u_all, const LbmLaunch& shape,
+                           hipStream_t s);
+// iters[i] += 1 for the n members
+void launch_advance_iters(int* iters, int n, hipStream_t s);
 
 }  // namespace glsm

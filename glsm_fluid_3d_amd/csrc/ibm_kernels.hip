@@ -102,9 +102,7 @@ __device__ inline int block_excl_scan(int v, int* s_wave, int& total) {
 
 // ---------------------------------------------------------------- coupler
 // k_gather_boundary, coupling_kernels.cu:42-59, all gels in one launch
-__global__ void __launch_bounds__(kThreads) k_pack(MarkerArrays m, float g2l) {
-  const int l = blockIdx.x * blockDim.x + threadIdx.x;
-  if (l >= m.M) return;
+__device__ __forceinline__ void pack_at(const MarkerArrays& m, float g2l, int l) {
   const GelMarkerDesc d = m.desc[m.owner[l]];
   const int id = m.bIndex[l];
   const double3 r = d.rn[id], v = d.Vels[id];
@@ -116,9 +114,7 @@ __global__ void __launch_bounds__(kThreads) k_pack(MarkerArrays m, float g2l) {
 
 // k_add_reaction_to_gel + k_add_drag_to_gel, coupling_kernels.cu:8-34;
 // every boundary node owns exactly one marker, so no two threads collide
-__global__ void __launch_bounds__(kThreads) k_scatter(MarkerArrays m) {
-  const int l = blockIdx.x * blockDim.x + threadIdx.x;
-  if (l >= m.M) return;
+__device__ __forceinline__ void scatter_at(const MarkerArrays& m, int l) {
   const GelMarkerDesc d = m.desc[m.owner[l]];
   const int id = m.bIndex[l];
   const float3 fr = m.Frep[l], fd = m.Fdrag[l];
@@ -137,10 +133,7 @@ __global__ void __launch_bounds__(kThreads) k_scatter(MarkerArrays m) {
 // wavefront per marker a; its 64 lanes evaluate 64 partners at a time, and
 // the partners inside the cut-off (a ballot mask) are added in increasing
 // index -- the reference's accumulation order -- by every lane alike.
-__global__ void __launch_bounds__(kThreads) k_repulsion(MarkerArrays m, float delta, float gamma) {
-  const int lane = threadIdx.x & 63;
-  const int a = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
-  if (a >= m.M) return;  // whole wavefront
+__device__ __forceinline__ void repulsion_wave(const MarkerArrays& m, float delta, float gamma, int a, int lane) {
   const float rcut = 2.5f * gamma, rc2 = rcut * rcut, sig2 = gamma * gamma;
   const float3 pa = m.lag[a];
   const int oa = m.owner[a];
@@ -180,11 +173,8 @@ __global__ void __launch_bounds__(kThreads) k_repulsion(MarkerArrays m, float de
 // ---------------------------------------------------------------- IBM
 // Ul = sum(u*phi)/sum(phi) over the stencil, Fdrag = beta_eff*(Vl - Ul)
 // (k_ibm_interpolate_velocity + k_scale_negbeta, fluid_kernels.cu:244-291)
-__global__ void __launch_bounds__(kThreads) k_ibm_interp(MarkerArrays m, SpreadLists sl, const float3* __restrict__ u,
-                                                         float beta_eff) {
-  const int lane = threadIdx.x & 63;
-  const int l = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
-  if (l >= m.M) return;  // whole wavefront
+__device__ __forceinline__ void interp_wave(const MarkerArrays& m, const SpreadLists& sl,
+                                            const float3* __restrict__ u, float beta_eff, int l, int lane) {
   const size_t k = (size_t)l * 64 + lane;
   const int cell = sl.ctab[k];
   float px = 0.f, py = 0.f, pz = 0.f;
@@ -214,10 +204,8 @@ __global__ void __launch_bounds__(kThreads) k_ibm_interp(MarkerArrays m, SpreadL
 
 // Dl = sum(c*phi)/sum(phi), Sl = Cl - Dl (k_ibm_sample_concentration +
 // k_robin_boundary, fluid_kernels.cu:350-387, 293-298)
-__global__ void __launch_bounds__(kThreads) k_ibm_sample(MarkerArrays m, SpreadLists sl, const float* __restrict__ c) {
-  const int lane = threadIdx.x & 63;
-  const int l = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
-  if (l >= m.M) return;
+__device__ __forceinline__ void sample_wave(const MarkerArrays& m, const SpreadLists& sl, const float* __restrict__ c,
+                                            int l, int lane) {
   const size_t k = (size_t)l * 64 + lane;
   const int cell = sl.ctab[k];
   const float p = cell >= 0 ? c[cell] * sl.rtab[k] : 0.f;
@@ -238,10 +226,8 @@ __global__ void __launch_bounds__(kThreads) k_ibm_sample(MarkerArrays m, SpreadL
 // fluid_kernels.cu:300-348); a marker whose weights sum to zero neither
 // interpolates nor spreads.  Counts contributions per cell and flags the
 // 256-cell blocks that receive any.
-__global__ void __launch_bounds__(kThreads) k_spread_count(LatticeGeom g, MarkerArrays m, SpreadLists sl) {
-  const int lane = threadIdx.x & 63;
-  const int l = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
-  if (l >= m.M) return;
+__device__ __forceinline__ void spread_count_wave(const LatticeGeom& g, const MarkerArrays& m, const SpreadLists& sl,
+                                                  int l, int lane) {
   const StencilLane st = stencil_lane(g, m.lag[l], lane);
   float ws = 0.f;
 #pragma unroll
@@ -262,15 +248,11 @@ __global__ void __launch_bounds__(kThreads) k_spread_count(LatticeGeom g, Marker
   sl.blkflag[st.cell / kFlagCells] = 1;
 }
 
-__global__ void k_spread_clear_flags(SpreadLists sl) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b < sl.nblk) sl.blkflag[b] = 0;
-}
+__device__ __forceinline__ void clear_flag_at(const SpreadLists& sl, int b) { sl.blkflag[b] = 0; }
 
 // per flagged block: total contributions
-__global__ void __launch_bounds__(kThreads) k_spread_block_sum(LatticeGeom g, SpreadLists sl) {
+__device__ __forceinline__ void block_sum_block(const LatticeGeom& g, const SpreadLists& sl, int b) {
   __shared__ int s_wave[kWavesPerBlock];
-  const int b = blockIdx.x;
   if (!sl.blkflag[b]) {
     if (threadIdx.x == 0) sl.bsum[b] = 0;
     return;
@@ -284,7 +266,7 @@ __global__ void __launch_bounds__(kThreads) k_spread_block_sum(LatticeGeom g, Sp
 
 // exclusive scan of the block totals (one work-group, tiles of 256 x 8
 // entries); bsum[nblk] = total
-__global__ void __launch_bounds__(kThreads) k_spread_scan_blocks(SpreadLists sl) {
+__device__ __forceinline__ void scan_blocks_block(const SpreadLists& sl) {
   __shared__ int s_wave[kWavesPerBlock];
   constexpr int kPer = 8;
   int carry = 0;
@@ -310,9 +292,8 @@ __global__ void __launch_bounds__(kThreads) k_spread_scan_blocks(SpreadLists sl)
 }
 
 // per flagged block: each cell's first entry and count; counts reset to 0
-__global__ void __launch_bounds__(kThreads) k_spread_local(LatticeGeom g, SpreadLists sl) {
+__device__ __forceinline__ void local_block(const LatticeGeom& g, const SpreadLists& sl, int b) {
   __shared__ int s_wave[kWavesPerBlock];
-  const int b = blockIdx.x;
   if (!sl.blkflag[b]) return;
   const int i = b * kFlagCells + threadIdx.x;
   const int v = i < g.N ? sl.cnt[i] : 0;
@@ -324,9 +305,7 @@ __global__ void __launch_bounds__(kThreads) k_spread_local(LatticeGeom g, Spread
   }
 }
 
-__global__ void __launch_bounds__(kThreads) k_spread_fill(int M64, SpreadLists sl) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= M64) return;
+__device__ __forceinline__ void fill_at(const SpreadLists& sl, int k) {
   const int cell = sl.ctab[k];
   if (cell < 0) return;
   const int pos = sl.rng[cell].x + atomicAdd(&sl.fillpos[cell], 1);
@@ -337,8 +316,7 @@ __global__ void __launch_bounds__(kThreads) k_spread_fill(int M64, SpreadLists s
 }
 
 // order each cell's entries by marker (insertion sort; a few dozen at most)
-__global__ void __launch_bounds__(kThreads) k_spread_sort(LatticeGeom g, SpreadLists sl) {
-  const int b = blockIdx.x;
+__device__ __forceinline__ void sort_at(const LatticeGeom& g, const SpreadLists& sl, int b) {
   if (!sl.blkflag[b]) return;
   const int i = b * kFlagCells + threadIdx.x;
   if (i >= g.N) return;
@@ -373,11 +351,11 @@ __device__ inline void decode_cell(const LatticeGeom& g, int i, int& x, int& y, 
 // first-order upwind advection + 7-point diffusion + spread IBM source,
 // forward Euler (k_zero_scalar + k_ibm_spread_concentration +
 // k_convection_diffusion, fluid_kernels.cu:119-123, 414-459, 177-242)
-__global__ void __launch_bounds__(kThreads) k_scalar(LatticeGeom g, ScalarParams p, const float3* __restrict__ u,
-                                                     const float* __restrict__ c1, float* __restrict__ c2,
-                                                     SpreadLists sl, const float* __restrict__ Sl) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int flagged = sl.blkflag[blockIdx.x];
+__device__ __forceinline__ void scalar_at(const LatticeGeom& g, const ScalarParams& p, const float3* __restrict__ u,
+                                          const float* __restrict__ c1, float* __restrict__ c2, const SpreadLists& sl,
+                                          const float* __restrict__ Sl, int b) {
+  const int i = b * kFlagCells + threadIdx.x;
+  const int flagged = sl.blkflag[b];
   if (i >= g.N) return;
   int x, y, z;
   decode_cell(g, i, x, y, z);
@@ -413,6 +391,155 @@ __global__ void __launch_bounds__(kThreads) k_scalar(LatticeGeom g, ScalarParams
   const float conv = -((FRx - FLx) * ihx + (FRy - FLy) * ihy + (FRz - FLz) * ihz);
   const float diff = p.D * ((cxm - 2.0f * c + cxp) * ihx2 + (cym - 2.0f * c + cyp) * ihy2 + (czm - 2.0f * c + czp) * ihz2);
   c2[i] = c + p.dt * (conv + diff + src);
+}
+
+// ---------------------------------------------------------------- kernels
+// Single-lattice kernels: one marker per thread / wavefront, one 256-cell
+// block per work-group.
+__global__ void __launch_bounds__(kThreads) k_pack(MarkerArrays m, float g2l) {
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l < m.M) pack_at(m, g2l, l);
+}
+__global__ void __launch_bounds__(kThreads) k_scatter(MarkerArrays m) {
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l < m.M) scatter_at(m, l);
+}
+__global__ void __launch_bounds__(kThreads) k_repulsion(MarkerArrays m, float delta, float gamma) {
+  const int a = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  if (a < m.M) repulsion_wave(m, delta, gamma, a, threadIdx.x & 63);  // whole wavefront
+}
+__global__ void __launch_bounds__(kThreads) k_ibm_interp(MarkerArrays m, SpreadLists sl, const float3* __restrict__ u,
+                                                         float beta_eff) {
+  const int l = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  if (l < m.M) interp_wave(m, sl, u, beta_eff, l, threadIdx.x & 63);
+}
+__global__ void __launch_bounds__(kThreads) k_ibm_sample(MarkerArrays m, SpreadLists sl, const float* __restrict__ c) {
+  const int l = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  if (l < m.M) sample_wave(m, sl, c, l, threadIdx.x & 63);
+}
+__global__ void __launch_bounds__(kThreads) k_spread_count(LatticeGeom g, MarkerArrays m, SpreadLists sl) {
+  const int l = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  if (l < m.M) spread_count_wave(g, m, sl, l, threadIdx.x & 63);
+}
+__global__ void k_spread_clear_flags(SpreadLists sl) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < sl.nblk) clear_flag_at(sl, b);
+}
+__global__ void __launch_bounds__(kThreads) k_spread_block_sum(LatticeGeom g, SpreadLists sl) {
+  block_sum_block(g, sl, blockIdx.x);
+}
+__global__ void __launch_bounds__(kThreads) k_spread_scan_blocks(SpreadLists sl) { scan_blocks_block(sl); }
+__global__ void __launch_bounds__(kThreads) k_spread_local(LatticeGeom g, SpreadLists sl) {
+  local_block(g, sl, blockIdx.x);
+}
+__global__ void __launch_bounds__(kThreads) k_spread_fill(int M64, SpreadLists sl) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < M64) fill_at(sl, k);
+}
+__global__ void __launch_bounds__(kThreads) k_spread_sort(LatticeGeom g, SpreadLists sl) { sort_at(g, sl, blockIdx.x); }
+__global__ void __launch_bounds__(kThreads) k_scalar(LatticeGeom g, ScalarParams p, const float3* __restrict__ u,
+                                                     const float* __restrict__ c1, float* __restrict__ c2,
+                                                     SpreadLists sl, const float* __restrict__ Sl) {
+  scalar_at(g, p, u, c1, c2, sl, Sl, blockIdx.x);
+}
+
+// Batched kernels: markers and 256-cell blocks of all members are numbered
+// consecutively (member i owns markers [mk0, mk0 + M) and blocks
+// [blk0, blk0 + nblk)); mk_inst / blk_inst map a global number to its member.
+__device__ __forceinline__ const BatchInst& marker_member(const BatchTables& t, int gm, int& l) {
+  const BatchInst& I = t.inst[t.mk_inst[gm]];
+  l = gm - I.mk0;
+  return I;
+}
+__device__ __forceinline__ const BatchInst& block_member(const BatchTables& t, int gb, int& b) {
+  const BatchInst& I = t.inst[t.blk_inst[gb]];
+  b = gb - I.blk0;
+  return I;
+}
+
+__global__ void __launch_bounds__(kThreads) kb_pack(BatchTables t) {
+  const int gm = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gm >= t.M) return;
+  int l;
+  const BatchInst& I = marker_member(t, gm, l);
+  pack_at(I.m, I.g2l, l);
+}
+__global__ void __launch_bounds__(kThreads) kb_scatter(BatchTables t) {
+  const int gm = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gm >= t.M) return;
+  int l;
+  const BatchInst& I = marker_member(t, gm, l);
+  scatter_at(I.m, l);
+}
+__global__ void __launch_bounds__(kThreads) kb_repulsion(BatchTables t) {
+  const int gm = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  if (gm >= t.M) return;
+  int l;
+  const BatchInst& I = marker_member(t, gm, l);
+  repulsion_wave(I.m, I.delta, I.gamma, l, threadIdx.x & 63);
+}
+// beta_eff = beta * min(1, (iter + 1) / ramp) in FP32 (fluid.cu:265-270)
+__device__ __forceinline__ float beta_eff_of(const BatchInst& I, int iter) {
+  const float ramp = (I.ramp <= 0.0f) ? 1.0f : fminf(1.0f, (float)(iter + 1) / I.ramp);
+  return I.beta * ramp;
+}
+__global__ void __launch_bounds__(kThreads) kb_ibm_interp(BatchTables t) {
+  const int gm = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  if (gm >= t.M) return;
+  int l;
+  const BatchInst& I = marker_member(t, gm, l);
+  interp_wave(I.m, I.sl, I.u, beta_eff_of(I, t.iters[t.mk_inst[gm]]), l, threadIdx.x & 63);
+}
+__global__ void __launch_bounds__(kThreads) kb_ibm_sample(BatchTables t, int cpar) {
+  const int gm = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  if (gm >= t.M) return;
+  int l;
+  const BatchInst& I = marker_member(t, gm, l);
+  sample_wave(I.m, I.sl, I.c[cpar], l, threadIdx.x & 63);
+}
+__global__ void __launch_bounds__(kThreads) kb_spread_count(BatchTables t) {
+  const int gm = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  if (gm >= t.M) return;
+  int l;
+  const BatchInst& I = marker_member(t, gm, l);
+  spread_count_wave(I.g, I.m, I.sl, l, threadIdx.x & 63);
+}
+__global__ void kb_spread_clear_flags(BatchTables t) {
+  const int gb = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gb >= t.nblk) return;
+  int b;
+  const BatchInst& I = block_member(t, gb, b);
+  clear_flag_at(I.sl, b);
+}
+__global__ void __launch_bounds__(kThreads) kb_spread_block_sum(BatchTables t) {
+  int b;
+  const BatchInst& I = block_member(t, blockIdx.x, b);
+  block_sum_block(I.g, I.sl, b);
+}
+__global__ void __launch_bounds__(kThreads) kb_spread_scan_blocks(BatchTables t) {
+  scan_blocks_block(t.inst[blockIdx.x].sl);  // one work-group per member
+}
+__global__ void __launch_bounds__(kThreads) kb_spread_local(BatchTables t) {
+  int b;
+  const BatchInst& I = block_member(t, blockIdx.x, b);
+  local_block(I.g, I.sl, b);
+}
+__global__ void __launch_bounds__(kThreads) kb_spread_fill(BatchTables t) {
+  const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= 64L * t.M) return;
+  int l;
+  const BatchInst& I = marker_member(t, (int)(k >> 6), l);
+  fill_at(I.sl, l * 64 + (int)(k & 63));
+}
+__global__ void __launch_bounds__(kThreads) kb_spread_sort(BatchTables t) {
+  int b;
+  const BatchInst& I = block_member(t, blockIdx.x, b);
+  sort_at(I.g, I.sl, b);
+}
+__global__ void __launch_bounds__(kThreads) kb_scalar(BatchTables t, int cpar) {
+  int b;
+  const BatchInst& I = block_member(t, blockIdx.x, b);
+  scalar_at(I.g, I.sp, I.u, I.c[cpar], I.c[cpar ^ 1], I.sl, I.m.Sl, b);
 }
 
 }  // namespace
@@ -458,6 +585,49 @@ void launch_spread_rebuild(const LatticeGeom& g, const MarkerArrays& m, const Sp
 void launch_scalar(const LatticeGeom& g, const ScalarParams& p, const float3* u, const float* c1, float* c2,
                    const SpreadLists& sl, const float* Sl, hipStream_t s) {
   k_scalar<<<blocks_for(g.N, kThreads), kThreads, 0, s>>>(g, p, u, c1, c2, sl, Sl);
+  GLSM_LAUNCH_CHECK();
+}
+
+// ---- batched launches (all members in one launch per phase) ----
+void launch_pack_batch(const BatchTables& t, hipStream_t s) {
+  if (t.M == 0) return;
+  kb_pack<<<blocks_for(t.M, kThreads), kThreads, 0, s>>>(t);
+  GLSM_LAUNCH_CHECK();
+}
+void launch_scatter_batch(const BatchTables& t, hipStream_t s) {
+  if (t.M == 0) return;
+  kb_scatter<<<blocks_for(t.M, kThreads), kThreads, 0, s>>>(t);
+  GLSM_LAUNCH_CHECK();
+}
+void launch_repulsion_batch(const BatchTables& t, hipStream_t s) {
+  if (t.M == 0) return;
+  kb_repulsion<<<blocks_for(t.M, kWavesPerBlock), kThreads, 0, s>>>(t);
+  GLSM_LAUNCH_CHECK();
+}
+void launch_ibm_interp_batch(const BatchTables& t, hipStream_t s) {
+  if (t.M == 0) return;
+  kb_ibm_interp<<<blocks_for(t.M, kWavesPerBlock), kThreads, 0, s>>>(t);
+  GLSM_LAUNCH_CHECK();
+}
+void launch_ibm_sample_batch(const BatchTables& t, int cpar, hipStream_t s) {
+  if (t.M == 0) return;
+  kb_ibm_sample<<<blocks_for(t.M, kWavesPerBlock), kThreads, 0, s>>>(t, cpar);
+  GLSM_LAUNCH_CHECK();
+}
+void launch_spread_rebuild_batch(const BatchTables& t, hipStream_t s) {
+  if (t.nblk == 0) return;
+  kb_spread_clear_flags<<<blocks_for(t.nblk, kThreads), kThreads, 0, s>>>(t);
+  if (t.M > 0) kb_spread_count<<<blocks_for(t.M, kWavesPerBlock), kThreads, 0, s>>>(t);
+  kb_spread_block_sum<<<t.nblk, kThreads, 0, s>>>(t);
+  kb_spread_scan_blocks<<<t.ninst, kThreads, 0, s>>>(t);
+  kb_spread_local<<<t.nblk, kThreads, 0, s>>>(t);
+  if (t.M > 0) kb_spread_fill<<<blocks_for(64L * t.M, kThreads), kThreads, 0, s>>>(t);
+  kb_spread_sort<<<t.nblk, kThreads, 0, s>>>(t);
+  GLSM_LAUNCH_CHECK();
+}
+void launch_scalar_batch(const BatchTables& t, int cpar, hipStream_t s) {
+  if (t.nblk == 0) return;
+  kb_scalar<<<t.nblk, kThreads, 0, s>>>(t, cpar);
   GLSM_LAUNCH_CHECK();
 }
 

@@ -56,13 +56,17 @@ __device__ __forceinline__ void put(T* p, T v) {
 }
 template <int S>
 struct Store;
+// st() stores v and returns the value kept, read back as ld() would.
 template <>
 struct Store<F32_FULL> {
   using T = float;
   static constexpr bool dev = false;
   __device__ static float ld(const T* p, float) { return *p; }
   template <bool NT>
-  __device__ static void st(T* p, float v, float) { put<T, NT>(p, v); }
+  __device__ static float st(T* p, float v, float, float) {
+    put<T, NT>(p, v);
+    return v;
+  }
 };
 template <>
 struct Store<F32_DEV> {
@@ -70,7 +74,10 @@ struct Store<F32_DEV> {
   static constexpr bool dev = true;
   __device__ static float ld(const T* p, float) { return *p; }
   template <bool NT>
-  __device__ static void st(T* p, float v, float) { put<T, NT>(p, v); }
+  __device__ static float st(T* p, float v, float, float) {
+    put<T, NT>(p, v);
+    return v;
+  }
 };
 template <>
 struct Store<F16_DEV> {
@@ -78,7 +85,11 @@ struct Store<F16_DEV> {
   static constexpr bool dev = true;
   __device__ static float ld(const T* p, float inv_scale) { return (float)*p * inv_scale; }
   template <bool NT>
-  __device__ static void st(T* p, float v, float scale) { put<T, NT>(p, (_Float16)(v * scale)); }
+  __device__ static float st(T* p, float v, float scale, float inv_scale) {
+    const _Float16 h = (_Float16)(v * scale);
+    put<T, NT>(p, h);
+    return (float)h * inv_scale;
+  }
 };
 template <>
 struct Store<BF16_DEV> {
@@ -86,10 +97,11 @@ struct Store<BF16_DEV> {
   static constexpr bool dev = true;
   __device__ static float ld(const T* p, float) { return __uint_as_float((uint32_t)*p << 16); }
   template <bool NT>
-  __device__ static void st(T* p, float v, float) {
+  __device__ static float st(T* p, float v, float, float) {
     const uint32_t b = __float_as_uint(v);
     const uint32_t r = (b + 0x7FFFu + ((b >> 16) & 1u)) >> 16;  // round to nearest even (finite v)
     put<T, NT>(p, (uint16_t)r);
+    return __uint_as_float(r << 16);
   }
 };
 
@@ -191,58 +203,94 @@ __device__ __forceinline__ float wq(const LbmParams& p, int q) { return q == 0 ?
 
 // BGK + Guo on one cell.  v[] holds f (full storage) or d = f - w (deviation
 // storage); the same representation is written back.
+// 16-bit storages: the moments are summed in FP64.  Every loaded value is a
+// 16-bit number times a power of two, and the 19 of a cell span far fewer
+// than 53 bits, so these sums are exact: rho and rho*u + F/2 are the exact
+// moments of the stored populations, each rounded once to FP32 for the
+// collision.  The collision conserves mass exactly (sum of the equilibria =
+// rho, the Guo term sums to 0), so the rest population is stored last as the
+// FP64 remainder of the cell's mass after the 18 moving ones were rounded:
+// a cell keeps its stored mass up to the one rounding of that remainder
+// instead of accumulating 19 roundings per sub-step.
 template <int S, bool FORCED, bool NT>
 __device__ __forceinline__ void collide_store(const float (&v)[19], float Fx, float Fy, float Fz, const LbmParams& p,
                                               typename Store<S>::T* __restrict__ fout, unsigned plane, unsigned i,
                                               float3* u, bool store_u) {
   constexpr bool DEV = Store<S>::dev;
-  float s = 0.f;
+  constexpr bool MOM64 = S == F16_DEV || S == BF16_DEV;
+  float s, rho, mx, my, mz;  // mass deviation (DEV) or mass, density, momentum incl. F/2
+  double sd = 0.0;           // MOM64: exact mass deviation
+  if (MOM64) {
 #pragma unroll
-  for (int q = 0; q < 19; ++q) s += v[q];
-  const float rho = DEV ? 1.0f + s : s;  // DEV: s = rho - 1
-  const float mx = (v[1] - v[2]) + (v[7] - v[8]) + (v[10] - v[9]) + (v[11] - v[12]) + (v[14] - v[13]);
-  const float my = (v[3] - v[4]) + (v[7] - v[10]) + (v[8] - v[9]) + (v[15] - v[17]) + (v[16] - v[18]);
-  const float mz = (v[5] - v[6]) + (v[11] - v[14]) + (v[12] - v[13]) + (v[15] - v[16]) + (v[18] - v[17]);
-  const float inv_rho = 1.0f / rho;
-  float ux, uy, uz;
-  if (FORCED) {
-    ux = (mx + 0.5f * Fx) * inv_rho;
-    uy = (my + 0.5f * Fy) * inv_rho;
-    uz = (mz + 0.5f * Fz) * inv_rho;
+    for (int q = 0; q < 19; ++q) sd += (double)v[q];
+    double mxd = ((double)v[1] - v[2]) + ((double)v[7] - v[8]) + ((double)v[10] - v[9]) +
+                 ((double)v[11] - v[12]) + ((double)v[14] - v[13]);
+    double myd = ((double)v[3] - v[4]) + ((double)v[7] - v[10]) + ((double)v[8] - v[9]) +
+                 ((double)v[15] - v[17]) + ((double)v[16] - v[18]);
+    double mzd = ((double)v[5] - v[6]) + ((double)v[11] - v[14]) + ((double)v[12] - v[13]) +
+                 ((double)v[15] - v[16]) + ((double)v[18] - v[17]);
+    if (FORCED) {
+      mxd += 0.5 * (double)Fx;
+      myd += 0.5 * (double)Fy;
+      mzd += 0.5 * (double)Fz;
+    }
+    s = (float)sd;
+    rho = (float)(1.0 + sd);
+    mx = (float)mxd;
+    my = (float)myd;
+    mz = (float)mzd;
   } else {
-    ux = mx * inv_rho;
-    uy = my * inv_rho;
-    uz = mz * inv_rho;
+    s = 0.f;
+#pragma unroll
+    for (int q = 0; q < 19; ++q) s += v[q];
+    rho = DEV ? 1.0f + s : s;  // DEV: s = rho - 1
+    mx = (v[1] - v[2]) + (v[7] - v[8]) + (v[10] - v[9]) + (v[11] - v[12]) + (v[14] - v[13]);
+    my = (v[3] - v[4]) + (v[7] - v[10]) + (v[8] - v[9]) + (v[15] - v[17]) + (v[16] - v[18]);
+    mz = (v[5] - v[6]) + (v[11] - v[14]) + (v[12] - v[13]) + (v[15] - v[16]) + (v[18] - v[17]);
+    if (FORCED) {
+      mx += 0.5f * Fx;
+      my += 0.5f * Fy;
+      mz += 0.5f * Fz;
+    }
   }
+  const float inv_rho = 1.0f / rho;
+  const float ux = mx * inv_rho, uy = my * inv_rho, uz = mz * inv_rho;
   if (store_u) u[i] = make_float3(ux, uy, uz);
+  // The multiply-adds below are written as fmaf: each rounds once, the same
+  // way in every kernel this body is inlined into.
   const float omega = p.omega;
-  const float uu15 = 1.5f * (ux * ux + uy * uy + uz * uz);
-  const float uF = FORCED ? ux * Fx + uy * Fy + uz * Fz : 0.f;
+  const float uu15 = 1.5f * fmaf(ux, ux, fmaf(uy, uy, uz * uz));
+  const float uF = FORCED ? fmaf(ux, Fx, fmaf(uy, Fy, uz * Fz)) : 0.f;
   const float kF = 1.0f - 0.5f * omega;
+  double kept = 0.0;  // MOM64: sum of the stored moving populations
   Unroll<19>::run([&](auto qc) {
     constexpr int q = decltype(qc)::value;
+    if (MOM64 && q == 0) return;  // stored last
     constexpr float cx = (float)CX[q], cy = (float)CY[q], cz = (float)CZ[q];
     const float w = wq(p, q);
-    const float cu = cx * ux + cy * uy + cz * uz;
-    const float poly = cu * (3.0f + 4.5f * cu) - uu15;
-    // feq = w rho (1 + poly); deviation form: feq - w = w (s + rho poly)
-    const float eq = DEV ? w * (s + rho * poly) : w * rho * (1.0f + poly);
-    float out = v[q] + omega * (eq - v[q]);
+    const float cu = cx * ux + cy * uy + cz * uz;  // c has components 0, +-1
+    const float poly = fmaf(cu, fmaf(4.5f, cu, 3.0f), -uu15);
+    // feq = w rho (1 + poly); deviation form: feq - w = w (s + rho poly);
+    // out = v + omega (feq - v)
+    const float neq = DEV ? fmaf(w, fmaf(rho, poly, s), -v[q]) : fmaf(w * rho, 1.0f + poly, -v[q]);
+    float out = fmaf(omega, neq, v[q]);
     if (FORCED) {
+      // Guo: w (1 - omega/2) (3 (c.F - u.F) + 9 (c.u)(c.F))
       const float ciF = cx * Fx + cy * Fy + cz * Fz;
-      out += w * kF * (3.0f * (ciF - uF) + 9.0f * cu * ciF);
+      out = fmaf(w * kF, fmaf(9.0f * cu, ciF, 3.0f * (ciF - uF)), out);
     }
-    Store<S>::template st<NT>(fout + (unsigned)q * plane + i, out, p.scale);
+    const float st = Store<S>::template st<NT>(fout + (unsigned)q * plane + i, out, p.scale, p.inv_scale);
+    if (MOM64) kept += (double)st;
   });
+  if (MOM64) Store<S>::template st<NT>(fout + i, (float)(sd - kept), p.scale, p.inv_scale);
 }
 
 template <int S, bool NT>
-__global__ void __launch_bounds__(kThreads) k_lbm_step(LatticeGeom g, LbmParams p,
-                                                       const typename Store<S>::T* __restrict__ fin,
-                                                       typename Store<S>::T* __restrict__ fout, unsigned plane,
-                                                       float3* __restrict__ u, int write_u_all, SpreadLists sl,
-                                                       const float3* __restrict__ Fl, int const_force) {
-  const unsigned b = xcd_block<NT ? 4 : 0>(blockIdx.x, gridDim.x);
+__device__ __forceinline__ void lbm_block(const LatticeGeom& g, const LbmParams& p,
+                                          const typename Store<S>::T* __restrict__ fin,
+                                          typename Store<S>::T* __restrict__ fout, unsigned plane,
+                                          float3* __restrict__ u, int write_u_all, const SpreadLists& sl,
+                                          const float3* __restrict__ Fl, int const_force, unsigned b) {
   const unsigned i = b * kThreads + threadIdx.x;
   const int flagged = sl.blkflag[b];
   if (i >= (unsigned)g.N) return;
@@ -274,6 +322,36 @@ __global__ void __launch_bounds__(kThreads) k_lbm_step(LatticeGeom g, LbmParams 
   }
 }
 
+template <int S, bool NT>
+__global__ void __launch_bounds__(kThreads) k_lbm_step(LatticeGeom g, LbmParams p,
+                                                       const typename Store<S>::T* __restrict__ fin,
+                                                       typename Store<S>::T* __restrict__ fout, unsigned plane,
+                                                       float3* __restrict__ u, int write_u_all, SpreadLists sl,
+                                                       const float3* __restrict__ Fl, int const_force) {
+  const unsigned b = xcd_block<NT ? 4 : 0>(blockIdx.x, gridDim.x);
+  lbm_block<S, NT>(g, p, fin, fout, plane, u, write_u_all, sl, Fl, const_force, b);
+}
+
+// All members of a batch: the blocks of every lattice form one flattened
+// index (member i's at [blk0, blk0 + nblk)); each work-group walks it with
+// stride gridDim.x (persistent when the grid is smaller than the block
+// count), the XCD remap applied to the flattened index.
+template <int S, bool NT, int RUN>
+__global__ void __launch_bounds__(kThreads) kb_lbm_step(BatchTables t, int sel, int write_u_all) {
+  using T = typename Store<S>::T;
+  for (unsigned w = blockIdx.x; w < (unsigned)t.nblk; w += gridDim.x) {
+    const unsigned gb = xcd_block<RUN>(w, (unsigned)t.nblk);
+    const BatchInst& I = t.inst[t.blk_inst[gb]];
+    lbm_block<S, NT>(I.g, I.p, static_cast<const T*>(I.f[sel]), static_cast<T*>(I.f[sel ^ 1]), (unsigned)I.plane,
+                     I.u, write_u_all, I.sl, I.m.Fdrag, I.const_force, gb - (unsigned)I.blk0);
+  }
+}
+
+__global__ void k_advance_iters(int* __restrict__ iters, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) iters[i] += 1;
+}
+
 // k_init, fluid_kernels.cu:84-96: f = w_q (a rest state, which is its own
 // post-collision state), u = 0, c1 = c2 = 0
 template <int S>
@@ -285,7 +363,7 @@ __global__ void __launch_bounds__(kThreads) k_lbm_init(LatticeGeom g, LbmParams 
   if (i >= (unsigned)g.N) return;
 #pragma unroll
   for (int q = 0; q < 19; ++q)
-    Store<S>::template st<false>(f + (unsigned)q * plane + i, Store<S>::dev ? 0.f : wq(p, q), p.scale);
+    Store<S>::template st<false>(f + (unsigned)q * plane + i, Store<S>::dev ? 0.f : wq(p, q), p.scale, p.inv_scale);
   u[i] = make_float3(0.f, 0.f, 0.f);
   c1[i] = 0.f;
   c2[i] = 0.f;
@@ -324,8 +402,74 @@ __global__ void __launch_bounds__(kThreads) k_lbm_import(LatticeGeom g, LbmParam
   Unroll<19>::run([&](auto qc) {
     constexpr int q = decltype(qc)::value;
     const float v = in[a.dst<q>(i, in_plane)];  // from plane q or opp q; w_opp(q) == w_q
-    Store<S>::template st<false>(fpost + (unsigned)q * plane + i, Store<S>::dev ? v - wq(p, q) : v, p.scale);
+    Store<S>::template st<false>(fpost + (unsigned)q * plane + i, Store<S>::dev ? v - wq(p, q) : v, p.scale, p.inv_scale);
   });
+}
+
+// ---- FP64 conservation reductions ----
+// Mass excess sum(rho - 1) and momentum sum(rho u) = sum_q f_q c_q of the
+// pre-collision populations of the next sub-step (what k_macros sums,
+// fluid_kernels.cu:98-111, without the F/2 term).  Per cell the 19 values
+// are summed in FP64; each work-group reduces its cells with wave shuffles
+// and a fixed-order combine of its 4 waves; k_reduce_partials adds the
+// per-group partials in a fixed order.  The result is deterministic.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+template <int K>
+__device__ __forceinline__ void block_sum_store(double (&acc)[K], double* __restrict__ out) {
+  __shared__ double part[kThreads / 64][K];
+  const unsigned lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < K; ++k) acc[k] = wave_sum(acc[k]);
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < K; ++k) part[wave][k] = acc[k];
+  __syncthreads();
+  if (threadIdx.x < K) {
+    double t = 0.0;
+#pragma unroll
+    for (int w = 0; w < kThreads / 64; ++w) t += part[w][threadIdx.x];
+    out[threadIdx.x] = t;
+  }
+}
+
+template <int S>
+__global__ void __launch_bounds__(kThreads) k_lbm_moments(LatticeGeom g, LbmParams p,
+                                                          const typename Store<S>::T* __restrict__ fpost,
+                                                          unsigned plane, double* __restrict__ partial) {
+  const unsigned i = blockIdx.x * kThreads + threadIdx.x;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  if (i < (unsigned)g.N) {
+    int x, y, z;
+    decode_cell(g, (int)i, x, y, z);
+    const PullAddr a(g, x, y, z);
+    double m = Store<S>::dev ? 0.0 : -1.0;
+    Unroll<19>::run([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      const double f = (double)Store<S>::ld(fpost + a.src<q>(i, plane), p.inv_scale);
+      // deviation storage: sum d = rho - 1 (the weights sum to 1), and
+      // sum d c = sum f c (the weights of opposite velocities are equal)
+      m += f;
+      if (CX[q]) acc[1] += CX[q] > 0 ? f : -f;
+      if (CY[q]) acc[2] += CY[q] > 0 ? f : -f;
+      if (CZ[q]) acc[3] += CZ[q] > 0 ? f : -f;
+    });
+    acc[0] = m;
+  }
+  block_sum_store<4>(acc, partial + 4ull * blockIdx.x);
+}
+
+__global__ void __launch_bounds__(kThreads) k_reduce_partials(const double* __restrict__ partial, int nblk,
+                                                              double* __restrict__ out) {
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int b = threadIdx.x; b < nblk; b += kThreads)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) acc[k] += partial[4ll * b + k];
+  block_sum_store<4>(acc, out);
 }
 
 // plain device copy (16 B per lane): the reference point for achievable HBM
@@ -388,6 +532,34 @@ void launch_lbm_step(int storage, const LatticeGeom& g, const LbmParams& p, cons
     default: throw std::invalid_argument("unknown storage mode");   \
   }
 
+template <int S>
+void step_batch_impl(const BatchTables& t, int sel, bool write_u_all, const LbmLaunch& sh, hipStream_t s) {
+  const int grid = sh.grid > 0 && sh.grid < t.nblk ? sh.grid : t.nblk;
+  const int wu = write_u_all ? 1 : 0;
+  if (sh.nt && sh.run)
+    kb_lbm_step<S, true, 4><<<grid, kThreads, 0, s>>>(t, sel, wu);
+  else if (sh.nt)
+    kb_lbm_step<S, true, 0><<<grid, kThreads, 0, s>>>(t, sel, wu);
+  else if (sh.run)
+    kb_lbm_step<S, false, 4><<<grid, kThreads, 0, s>>>(t, sel, wu);
+  else
+    kb_lbm_step<S, false, 0><<<grid, kThreads, 0, s>>>(t, sel, wu);
+}
+
+void launch_lbm_step_batch(int storage, const BatchTables& t, int sel, bool write_u_all, const LbmLaunch& shape,
+                           hipStream_t s) {
+  if (t.nblk == 0) return;
+  if (shape.run != 0 && shape.run != 4) throw std::invalid_argument("XCD run length must be 0 or 4");
+  GLSM_STORAGE_DISPATCH(storage, (step_batch_impl<S>(t, sel, write_u_all, shape, s)));
+  GLSM_LAUNCH_CHECK();
+}
+
+void launch_advance_iters(int* iters, int n, hipStream_t s) {
+  if (n == 0) return;
+  k_advance_iters<<<blocks_for(n), kThreads, 0, s>>>(iters, n);
+  GLSM_LAUNCH_CHECK();
+}
+
 void launch_lbm_init(int storage, const LatticeGeom& g, const LbmParams& p, void* f, size_t plane, float3* u,
                      float* c1, float* c2, hipStream_t s) {
   GLSM_STORAGE_DISPATCH(storage, (k_lbm_init<S><<<blocks_for(g.N), kThreads, 0, s>>>(
@@ -406,6 +578,17 @@ void launch_lbm_export(int storage, bool pull, const LatticeGeom& g, const LbmPa
                                        g, p, static_cast<const typename Store<S>::T*>(fpost), out, (unsigned)plane,
                                        (unsigned)out_plane)));
   }
+  GLSM_LAUNCH_CHECK();
+}
+
+void launch_lbm_moments(int storage, const LatticeGeom& g, const LbmParams& p, const void* fpost, size_t plane,
+                        double* partial, double* out4, hipStream_t s) {
+  const int nb = blocks_for(g.N);
+  GLSM_STORAGE_DISPATCH(storage, (k_lbm_moments<S><<<nb, kThreads, 0, s>>>(
+                                     g, p, static_cast<const typename Store<S>::T*>(fpost), (unsigned)plane,
+                                     partial)));
+  GLSM_LAUNCH_CHECK();
+  k_reduce_partials<<<1, kThreads, 0, s>>>(partial, nb, out4);
   GLSM_LAUNCH_CHECK();
 }
 

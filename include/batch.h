@@ -1,0 +1,102 @@
+// batch.h -- many independent gel-fluid instances stepped as one.
+//
+// Extension with no counterpart in the reference (whose drivers step one
+// instance: sim.cpp:60-111).  A Batch holds instances made of the
+// reference's own objects -- a gel vector, its Coupler and its Fluid -- and
+// advances all of them by one time step with one launch per phase: the
+// gel phases over every gel, the marker phases over every marker, and the
+// 25 stream/collide sub-steps as one persistent launch each over the
+// concatenated 256-cell blocks of every lattice.  Each instance gives the
+// same bits as stepping its objects alone (sim.cpp's loop body); only the
+// launch shape differs.
+//
+// The step can be captured in a HIP graph (one per buffer parity) and the
+// stream/collide launch shape chosen by timing the candidates.
+#pragma once
+
+#include <map>
+#include <memory>
+#include <vector>
+
+#include "coupling.h"
+#include "fluid.h"
+#include "gel.h"
+
+namespace glsm {
+struct BatchDevice;
+}
+
+class Batch {
+ public:
+  // storage: the distribution storage of every member (Fluid::Storage)
+  explicit Batch(int storage = Fluid::F32_DEV, int scale_exp = 20);
+  ~Batch();
+  Batch(const Batch&) = delete;
+  Batch& operator=(const Batch&) = delete;
+
+  // Adds an instance.  The objects stay owned by the caller and must outlive
+  // its membership; `gels` must be the vector the coupler was built from.
+  // The fluid is converted to the batch's storage.  `iter` is the iteration
+  // of the instance's next step.  With write_files the .dat output of
+  // Gel::writeFiles / Fluid::writeFiles is produced at the reference's
+  // cadence after each step.  Returns the member id.
+  int add(std::vector<Gel*>& gels, Coupler* coupler, Fluid* fluid, int iter, bool write_files = false);
+  void remove(int id);
+  int size() const { return (int)members_.size(); }
+  std::vector<int> ids() const;
+  int iteration(int id) const;  // the member's next iteration
+  long cells() const;           // lattice cells over all members
+
+  // `nsteps` time steps of every member (sim.cpp:66-81 each).
+  void step(int nsteps = 1);
+
+  // Capture each step in a HIP graph (default on).
+  void useGraph(bool on);
+
+  // Launch shape of the batched stream/collide kernel.
+  struct Launch {
+    int nt;    // non-temporal stores
+    int run;   // XCD run length: 0 (one contiguous run per XCD) or 4
+    int grid;  // work-groups; 0 = one per 256-cell block, else persistent
+  };
+  Launch launch() const;
+  void setLaunch(const Launch& l);
+  // Times every candidate shape on the members' current state (state is
+  // not changed) and keeps the fastest; returns it and its mean launch time.
+  Launch autotune(int reps, double* best_ms);
+
+  // Mean duration of the batched stream/collide launch (HIP events around
+  // the middle sub-step of every `every`-th step; 0 = off).
+  void enableLbmTiming(int every);
+  double lbmKernelTiming(int* samples);
+
+ private:
+  struct Member {
+    int id;
+    std::vector<Gel*>* gels;
+    Coupler* coupler;
+    Fluid* fluid;
+    int iter;
+    bool write_files;
+  };
+  void rebuild();
+  void enqueueStep(bool timed);
+  void dropGraphs();
+
+  int storage_;
+  int scale_exp_;
+  int next_id_ = 1;
+  std::vector<Member> members_;
+  bool dirty_ = true;
+  bool use_graph_ = true;
+  bool launch_set_ = false;
+  int pf_ = 0;  // which of a member's two distribution buffers holds its state
+  int pc_ = 0;  // which of its two concentration buffers holds its state
+  int nsub_ = 0;
+  Launch launch_{0, 0, 0};
+  std::unique_ptr<glsm::BatchDevice> dev_;
+  std::map<int, void*> graphs_;  // (pf, pc) -> hipGraphExec_t
+  int timing_every_ = 0;
+  long timing_calls_ = 0;
+  std::vector<void*> timing_events_;
+};

@@ -37,6 +37,8 @@ class Coupler {
   // the previous pack (some gel moved); the fluid re-bins its spreading
   // lists only then.
   uint64_t lagVersion() const { return m_lag_version; }
+  // device table of the gels' boundary-node arrays (one entry per gel)
+  const GelMarkerDesc* deviceGelTable() const { return d_desc_; }
 
   CouplerParams* h_cp;
   CouplerParams* d_cp;

@@ -78,6 +78,7 @@ class Fluid {
   enum Storage { F32_FULL = 0, F32_DEV = 1, F16_DEV = 2, BF16_DEV = 3 };
   void setStorage(int storage, int scale_exp = 20);
   int storage() const { return storage_; }
+  int storageScaleExp() const { return scale_exp_; }
 
   // Measurement hook: when enabled, every `every`-th stepVelocity brackets
   // its middle sub-step's stream/collide launch with a HIP event pair on the
@@ -86,11 +87,19 @@ class Fluid {
   void enableLbmTiming(int every, int max_samples = 4096);
   double lbmKernelTiming(int* samples);
 
+  // FP64 conservation reductions over the pre-collision populations of the
+  // next sub-step: out = {sum(rho - 1), sum(rho ux), sum(rho uy), sum(rho uz)}
+  // with rho u = sum_q f_q c_q (fluid_kernels.cu:98-111 without F/2).
+  // Deterministic; waits for the fluid stream.
+  void conservation(double out[4]);
+
   // Pre-collision populations f_k(x,q) (what the reference's d_f holds) and
   // their inverse; host arrays of 19*N floats, plane-major.
   void readPopulations(float* host19N);
   void writePopulations(const float* host19N);
   void readPostCollision(float* host19N);
+  // device view of the IBM spreading lists (valid after the first step)
+  glsm::SpreadLists spreadLists() const;
 
   // host data
   float3* h_u;
@@ -125,7 +134,6 @@ class Fluid {
 
  private:
   void ensureSpreadLists();
-  glsm::SpreadLists spreadLists() const;
   int storage_ = F32_DEV;
   int scale_exp_ = 20;
   int timing_every_ = 0;
@@ -138,6 +146,7 @@ class Fluid {
   int* d_fillpos_ = nullptr;  // per-cell fill cursor (scratch, kept 0)
   int2* d_rng_ = nullptr;     // per-cell (first entry, count), flagged blocks
   int* d_bsum_ = nullptr;     // per-block entry totals -> offsets
+  double* d_red_ = nullptr;   // 4 FP64 partials per block + the 4 totals
   SpreadEntry* d_ent_ = nullptr;
   int* d_ctab_ = nullptr;     // per (marker, stencil lane): cell or -1
   float* d_wtab_ = nullptr;   // per (marker, stencil lane): spreading weight

@@ -58,6 +58,10 @@ class Gel {
 
   // Extension: seeded additive chemistry perturbation (splitmix64).
   void perturbChemistry(uint64_t seed, double amplitude);
+  // Extension: rate constants f and ep (GelParams, gel.cu:633-652) of this
+  // gel; the steady state is solved again (gel.cu:424-443) and the interior
+  // chemistry set to it, as at construction (gel.cu:179-219).
+  void setChemistryRates(double f, double ep);
 
   int get_index(int xi, int yi, int zi, int size);  // gel.cu:116-119
   int idx3(int x, int y, int z, int Nx, int Ny);    // gel.cu:156-159
@@ -72,6 +76,7 @@ class Gel {
   // Monotone counter bumped whenever node positions may have changed; the
   // coupler uses it to skip re-binning unchanged IBM markers.
   uint64_t positionVersion() const { return m_rn_version; }
+  void markPositionsChanged() { ++m_rn_version; }
 
   // ---- host data (reference names) ----
   GelParams* m_hgp;

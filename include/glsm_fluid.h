@@ -93,6 +93,13 @@ int glsm_gel_step_elasticity(glsm_gel* g, int iter);        /* gel.cu:682-698 */
 int glsm_gel_step_chemistry(glsm_gel* g, int iter);         /* gel.cu:700-704 */
 int glsm_gel_record_center(glsm_gel* g, int iter);          /* gel.cu:706-713 */
 int glsm_gel_write_files(glsm_gel* g, int iter);            /* gel.cu:543-553 */
+/* Extension: lossless binary snapshot of every device field, the params and
+   the output flags.  It complements the reference's text restart
+   Gel::setGoonValue (gel.cu:221-282), which keeps 6 decimals of rn/um and 6
+   significant digits of vm/wm and rebuilds the rest.  Loading needs a gel of
+   the same size and type; on a failed load the gel's state is unspecified. */
+int glsm_gel_save_state(glsm_gel* g, const char* path);
+int glsm_gel_load_state(glsm_gel* g, const char* path);
 int glsm_gel_reset_to_quiescent(glsm_gel* g);               /* gel.cu:731-747 */
 int glsm_gel_fire_excitation_pulse(glsm_gel* g);            /* gel.cu:751-783 */
 int glsm_gel_set_anchor_z(glsm_gel* g, double anchor_z);    /* gel.cu:788-795 */
@@ -104,6 +111,10 @@ int glsm_gel_dims(glsm_gel* g, int node_grid[3], int* num_nodes, int* num_elemen
 /* Extension (no reference counterpart): interior elements in z,y,x order get
    u += amp*U(-1,1), then v += amp*U(-1,1), drawn from splitmix64(seed). */
 int glsm_gel_perturb_chemistry(glsm_gel* g, uint64_t seed, double amplitude);
+/* Extension: rate constants f and ep (GelParams, gel.cu:633-652); the steady
+   state is solved again (gel.cu:424-443) and the interior chemistry set to
+   it (gel.cu:179-219).  Call before stepping. */
+int glsm_gel_set_chemistry_rates(glsm_gel* g, double f, double ep);
 size_t glsm_gel_field_bytes(glsm_gel* g, int field);
 int glsm_gel_read(glsm_gel* g, int field, void* host, size_t bytes);
 int glsm_gel_write(glsm_gel* g, int field, const void* host, size_t bytes);
@@ -130,6 +141,19 @@ int glsm_fluid_step_velocity(glsm_fluid* f, int iter);      /* fluid.cu:263-284 
 int glsm_fluid_sub_step(glsm_fluid* f, float beta_eff);
 int glsm_fluid_step_concentration(glsm_fluid* f, int iter); /* fluid.cu:251-261 */
 int glsm_fluid_write_files(glsm_fluid* f, int iter);        /* fluid.cu:140-150 */
+/* Extension (precision form): FP64 conservation reductions over the
+   pre-collision populations of the next sub-step (the reference's d_f):
+   out[0] = sum over cells of (rho - 1), out[1..3] = sum of rho*u with
+   rho*u = sum_q f_q c_q as in k_macros (fluid_kernels.cu:98-111) without the
+   F/2 term.  Per-cell sums in FP64, fixed-order block reduction. */
+int glsm_fluid_conservation(glsm_fluid* f, double out[4]);
+/* Extension: lossless binary snapshot of the stored populations (in their
+   storage representation), u, c and the params; loading switches the storage
+   mode to the saved one.  It complements Fluid::setGoonValue
+   (fluid.cu:57-83), which restarts f at rest and reads u and c as text.  The
+   fluid must have the same lattice and marker count. */
+int glsm_fluid_save_state(glsm_fluid* f, const char* path);
+int glsm_fluid_load_state(glsm_fluid* f, const char* path);
 /* FluidParams::perX/perY/perZ (fluid.h:21); honoured by stream, IBM and scalar */
 int glsm_fluid_set_periodic(glsm_fluid* f, int per_x, int per_y, int per_z);
 /* FluidParams::beta and the IBM ramp length, normally from $IBM_BETA /
@@ -161,6 +185,35 @@ int glsm_fluid_write(glsm_fluid* f, int field, const void* host, size_t bytes);
    (bytes read + bytes written) / s in GB/s.  The reference point for the
    roofline and for calibrating HBM traffic counters. */
 int glsm_copy_bandwidth(size_t bytes, int reps, double* gbps);
+
+/* ---- Batch (include/batch.h; no reference counterpart): many independent
+   instances, each a coupler's gels + the coupler + its fluid, advanced one
+   time step at a time by one launch per phase over all of them (HIP-graph
+   captured).  Every instance gives the same bits as glsm_system_step on its
+   own objects.  The objects stay owned by the caller and must outlive their
+   membership. ---- */
+typedef struct glsm_batch glsm_batch;
+int glsm_batch_create(int storage, int scale_exp, glsm_batch** out);
+int glsm_batch_destroy(glsm_batch* b);
+/* iter = the instance's next iteration; write_files = produce the .dat
+   output (Gel::writeFiles / Fluid::writeFiles cadence) after each step.
+   Returns the member id (> 0) or a negative status. */
+int glsm_batch_add(glsm_batch* b, glsm_coupler* c, glsm_fluid* f, int iter, int write_files);
+int glsm_batch_remove(glsm_batch* b, int id);
+int glsm_batch_size(glsm_batch* b);
+int glsm_batch_iteration(glsm_batch* b, int id);
+int glsm_batch_step(glsm_batch* b, int nsteps);
+int glsm_batch_use_graph(glsm_batch* b, int on);
+/* stream/collide launch shape: nt = non-temporal stores, run = XCD run length
+   (0 or 4), grid = work-groups (0 = one per 256-cell block, else
+   persistent) */
+int glsm_batch_set_launch(glsm_batch* b, int nt, int run, int grid);
+int glsm_batch_get_launch(glsm_batch* b, int* nt, int* run, int* grid);
+/* time every launch shape on the current state (unchanged) and keep the
+   fastest; best_ms = its mean stream/collide launch time */
+int glsm_batch_autotune(glsm_batch* b, int reps, double* best_ms);
+int glsm_batch_enable_lbm_timing(glsm_batch* b, int every);
+int glsm_batch_lbm_timing(glsm_batch* b, double* mean_ms, int* samples);
 
 /* ---- one time step in sim.cpp's order (sim.cpp:66-81): for each gel
    stepElasticity, stepChemistry, recordCenter; packFromGels; stepVelocity;

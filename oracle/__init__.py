@@ -97,6 +97,7 @@ def lib():
             "oracle_gel_set_anchor_z": (None, [vp, d]),
             "oracle_gel_build_tube_mask": (None, [vp, i, i]),
             "oracle_gel_perturb_chemistry": (None, [vp, C.c_uint64, d]),
+            "oracle_gel_set_chemistry_rates": (None, [vp, d, d]),
             "oracle_gel_dims": (None, [vp, C.POINTER(i), C.POINTER(i), C.POINTER(i), C.POINTER(i)]),
             "oracle_gel_params": (None, [vp, C.POINTER(d), C.POINTER(d), C.POINTER(d)]),
             "oracle_gel_read": (i, [vp, i, vp, C.c_size_t]),
@@ -187,6 +188,9 @@ class Gel:
 
     def perturb_chemistry(self, seed, amplitude):
         lib().oracle_gel_perturb_chemistry(self.h, int(seed), float(amplitude))
+
+    def set_chemistry_rates(self, f, ep):
+        lib().oracle_gel_set_chemistry_rates(self.h, float(f), float(ep))
 
     def steady_state(self):
         u, v, w = C.c_double(), C.c_double(), C.c_double()

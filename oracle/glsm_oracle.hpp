@@ -359,6 +359,23 @@ class Gel {
     um = h_um; vm = h_vm; wm = h_wm;
   }
 
+  // Extension: rate constants f, ep; steady state solved again
+  // (gel.cu:424-443) and the interior chemistry set to it (gel.cu:179-219).
+  void setChemistryRates(double f, double ep) {
+    gp.f = f;
+    gp.ep = ep;
+    steadyStateValue(gp.uss, gp.vss, gp.wss, 0);
+    for (int zi = 1; zi < L.z; zi++)
+      for (int yi = 1; yi < L.y; yi++)
+        for (int xi = 1; xi < L.x; xi++) {
+          int gi = elem(xi, yi, zi);
+          um[gi] = gp.uss;
+          vm[gi] = gp.vss;
+          wm[gi] = gp.wss;
+        }
+    snapshotToHost();
+  }
+
   // Extension: interior elements (z,y,x order) get u += amp*U(-1,1) then
   // v += amp*U(-1,1) from splitmix64(seed); w untouched.
   void perturbChemistry(uint64_t seed, double amp) {

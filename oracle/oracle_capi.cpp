@@ -175,6 +175,9 @@ void oracle_gel_build_tube_mask(void* g, int wall, int circular) {
 void oracle_gel_perturb_chemistry(void* g, uint64_t seed, double amp) {
   static_cast<Gel*>(g)->perturbChemistry(seed, amp);
 }
+void oracle_gel_set_chemistry_rates(void* g, double f, double ep) {
+  static_cast<Gel*>(g)->setChemistryRates(f, ep);
+}
 void oracle_gel_dims(void* gv, int* grid3, int* nN, int* nE, int* M) {
   Gel* g = static_cast<Gel*>(gv);
   grid3[0] = g->L.x; grid3[1] = g->L.y; grid3[2] = g->L.z;

@@ -84,6 +84,34 @@ def test_steady_state_is_the_model_fixed_point(oracle):
     assert abs(uss - 0.075442021435) < 1e-11 and abs(vss - 0.064955580456) < 1e-11
 
 
+@pytest.mark.parametrize("f,ep", [(0.72, 0.24), (1.08, 0.36)])
+def test_varied_rates_steady_state_and_step(oracle, f, ep):
+    """Rate constants moved by +-20%: the steady state is the root of the
+    model's u-nullcline at that f (independent Brent root), the interior is
+    reset to it, and one chemistry step applies ep and f as the model does."""
+    from scipy.optimize import brentq
+
+    g = oracle.Gel((3, 3, 3), (5, 5, 5))
+    g.set_chemistry_rates(f, ep)
+    uss, vss, wss = g.steady_state()
+    assert wss == 0.139 and abs(uss - vss / (1 - wss)) < 1e-15
+    v_star = brentq(lambda v: fu_model(v / (1 - wss), v, wss, f=f), 1e-3, 0.9, xtol=1e-15)
+    assert abs(vss - v_star) < 1e-9
+    L = g.node_grid
+    interior = [x + y * (L[0] + 1) + z * (L[0] + 1) * (L[1] + 1)
+                for z in range(1, L[2]) for y in range(1, L[1]) for x in range(1, L[0])]
+    assert np.all(g.read("um")[interior] == uss) and np.all(g.read("vm")[interior] == vss)
+    um = g.read("um")
+    vm = g.read("vm")
+    um[interior] += 0.01
+    g.write("um", um)
+    g.step_chemistry(1)  # T0 = T1 = T2 = 0 before any elasticity step
+    exp_u = um[interior] + 1e-3 * fu_model(um[interior], vm[interior], wss, f=f)
+    exp_v = vm[interior] + 1e-3 * ep * fv_model(um[interior], vm[interior], wss)
+    assert np.abs(g.read("um")[interior] - exp_u).max() < 1e-15
+    assert np.abs(g.read("vm")[interior] - exp_v).max() < 1e-15
+
+
 def test_chemistry_forward_euler_step(oracle):
     g = oracle.Gel((4, 3, 2), (5, 5, 5))
     nE = g.num_elements
