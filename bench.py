@@ -75,66 +75,10 @@ def config2b(mod, storage):
     return gels, c, f
 
 
-def round_to_odd(x):
-    return int(2 * round((x - 1) / 2) + 1)
-
-
-def place_gels(rng, box, sizes, margin=2.0, tries=200):
-    """Seeded centres for cube gels of the given element counts inside a
-    cubic box, at least `margin` from the walls and one element apart from
-    each other (boxes separated along some axis); gels that find no place in
-    `tries` draws are dropped."""
-    out = []
-    for n in sizes:
-        half = n / 2.0
-        lo, hi = half + margin, box - half - margin
-        if hi <= lo:
-            continue
-        for _ in range(tries):
-            p = rng.uniform(lo, hi, 3)
-            if all(np.any(np.abs(p - q) >= half + m / 2.0 + 1.0) for q, m in out):
-                out.append((p, n))
-                break
-    return out
-
-
-def batch_specs(count=64, seed=3, edge_range=(33, 97), gels_range=(1, 4), n_range=(4, 8), rate_jitter=0.2):
-    """Independent instances of the batching workload: lattice edge
-    log-uniform in edge_range rounded to odd (box = (edge - 1) h, h = 0.5),
-    1-4 cube gels of 4^3-8^3 elements at seeded non-overlapping positions,
-    rate constants f, ep uniform within +-20% of 0.9, 0.3."""
-    rng = np.random.default_rng(seed)
-    specs = []
-    for k in range(count):
-        edge = round_to_odd(np.exp(rng.uniform(np.log(edge_range[0]), np.log(edge_range[1]))))
-        box = (edge - 1) * 0.5
-        ng = int(rng.integers(gels_range[0], gels_range[1] + 1))
-        sizes = [int(rng.integers(n_range[0], n_range[1] + 1)) for _ in range(ng)]
-        placed = place_gels(rng, box, sizes)
-        gels = [(n, tuple(float(v) for v in p), 0.9 * rng.uniform(1 - rate_jitter, 1 + rate_jitter),
-                 0.3 * rng.uniform(1 - rate_jitter, 1 + rate_jitter)) for p, n in placed]
-        specs.append({"edge": edge, "box": box, "gels": gels, "seed": seed * 1000 + k})
-    return specs
-
-
-def build_instance(mod, spec, storage=None):
-    gels = []
-    for j, (n, pos, f, ep) in enumerate(spec["gels"]):
-        g = mod.Gel((n, n, n), pos, gel_id=j + 1)
-        g.set_chemistry_rates(f, ep)
-        g.perturb_chemistry(spec["seed"] * 8 + j, 0.01)
-        gels.append(g)
-    c = mod.Coupler(gels)
-    b = spec["box"]
-    f = mod.Fluid((b, b, b), c)
-    if storage:
-        f.set_storage(storage)
-    return gels, c, f
-
-
 def run_batch(G, storage, steps, warmup, count=64, tune=True):
-    specs = batch_specs(count)
-    insts = [build_instance(G, sp, storage) for sp in specs]
+    from glsm_fluid_3d_amd import workloads as W
+
+    insts = [W.build(G, sp, storage) for sp in W.batch_instances(count)]
     b = G.Batch(storage)
     for inst in insts:
         b.add(*inst)
@@ -162,6 +106,31 @@ def run_batch(G, storage, steps, warmup, count=64, tune=True):
         for x in gels + [f, c]:
             x.close()
     return out
+
+
+def run_serve(G, seconds, outputs=True, seed=4):
+    """The serving workload for `seconds` of wall time: Poisson arrivals at
+    20 jobs/s, edge 41-161, 1-6 gels of 6^3, horizons 1000-20000 steps, FP16
+    storage, .dat output streamed asynchronously (into a scratch directory
+    removed afterwards)."""
+    import shutil
+    import tempfile
+
+    from glsm_fluid_3d_amd import serving as S
+    from glsm_fluid_3d_amd import workloads as W
+
+    jobs = W.poisson_jobs(seed=seed, duration=seconds)
+    root = tempfile.mkdtemp(prefix="glsm_serve_") if outputs else None
+    srv = S.Server(G, storage="f16", out_root=root, keep_output=False, writers=8)
+    try:
+        stats = srv.run(jobs, duration=seconds)
+    finally:
+        srv.close()
+        if root:
+            shutil.rmtree(root, ignore_errors=True)
+    stats["arrived"] = len(jobs)
+    stats["offered_MLUPS"] = round(sum(j.cells * 25 * j.horizon for j in jobs) / seconds / 1e6, 1)
+    return stats
 
 
 def run_gpu(G, gels, c, f, steps, warmup, every=10):
@@ -229,8 +198,12 @@ def main():
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--extra-steps", type=int, default=400)
     ap.add_argument("--batch-steps", type=int, default=300)
+    ap.add_argument("--serve-seconds", type=float, default=45.0)
+    ap.add_argument("--interp-mode", type=int, default=0, choices=[0, 1, 2],
+                    help="IBM interpolation form: 0 by marker count, 1 wavefront per marker, 2 thread per marker")
     ap.add_argument("--workload", default="all",
-                    choices=["all", "101", "201-f32", "201-f16", "201-bf16", "copy", "batch-f32", "batch-f16"],
+                    choices=["all", "101", "201-f32", "201-f16", "201-bf16", "copy", "batch-f32", "batch-f16",
+                             "serve"],
                     help="profile one workload alone (no CPU baseline); 'copy' = the 1.23 GB calibration copy")
     args = ap.parse_args()
     if args.workload != "all":
@@ -244,10 +217,16 @@ def main():
     import glsm_fluid_3d_amd as G
 
     G.init(0)
+    G.set_interp_mode(args.interp_mode)
     if args.workload == "copy":
         nbytes = 2 * 19 * 4 * 8120601 // 2  # one 201^3 FP32 distribution buffer
         print(json.dumps({"workload": "plain 16-B copy", "bytes": nbytes,
                           "GBs": round(G.copy_bandwidth(nbytes, max(1, args.steps)), 1)}), flush=True)
+        return
+    if args.workload == "serve":
+        r = run_serve(G, args.serve_seconds)
+        r["workload"] = f"Poisson 20 jobs/s for {args.serve_seconds:.0f} s, f16, async .dat output"
+        print(json.dumps(r), flush=True)
         return
     if args.workload.startswith("batch"):
         st = args.workload.split("-")[1]
@@ -308,6 +287,10 @@ def main():
             extra[f"batch64_{st}"] = r
             log(f"batch of 64 {st}: {r['value']:.0f} MLUPS, lbm kernel {r['roofline']['mean_launch_ms'] * 1e3:.1f} us, "
                 f"launch {r['launch']}")
+        r = run_serve(G, args.serve_seconds)
+        extra["serve_poisson20_f16"] = r
+        log(f"serving {args.serve_seconds:.0f} s: {r['completed']} done, p50 {r['latency_p50_s']} s, "
+            f"p99 {r['latency_p99_s']} s, {r['mlups']:.0f} MLUPS")
         line["extra"] = extra
 
     if not args.no_cpu_baseline:

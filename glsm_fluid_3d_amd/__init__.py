@@ -85,9 +85,10 @@ SIGNATURES = {
     "glsm_fluid_field_bytes": (_sz, [_vp, _i]),
     "glsm_fluid_read": (_i, [_vp, _i, _vp, _sz]),
     "glsm_fluid_write": (_i, [_vp, _i, _vp, _sz]),
+    "glsm_set_interp_mode": (_i, [_i]),
     "glsm_batch_create": (_i, [_i, _i, C.POINTER(_vp)]),
     "glsm_batch_destroy": (_i, [_vp]),
-    "glsm_batch_add": (_i, [_vp, _vp, _vp, _i, _i]),
+    "glsm_batch_add": (_i, [_vp, _vp, _vp, _i, _i, C.c_char_p]),
     "glsm_batch_remove": (_i, [_vp, _i]),
     "glsm_batch_size": (_i, [_vp]),
     "glsm_batch_iteration": (_i, [_vp, _i]),
@@ -98,6 +99,15 @@ SIGNATURES = {
     "glsm_batch_autotune": (_i, [_vp, _i, C.POINTER(_d)]),
     "glsm_batch_enable_lbm_timing": (_i, [_vp, _i]),
     "glsm_batch_lbm_timing": (_i, [_vp, C.POINTER(_d), C.POINTER(_i)]),
+    "glsm_output_create": (_i, [_i, C.POINTER(_vp)]),
+    "glsm_output_destroy": (_i, [_vp]),
+    "glsm_output_gel": (_i, [_vp, _vp, _i, C.c_char_p]),
+    "glsm_output_fluid": (_i, [_vp, _vp, _i, C.c_char_p]),
+    "glsm_output_release": (_i, [_vp, _vp]),
+    "glsm_output_flush": (_i, [_vp]),
+    "glsm_output_status": (_i, [_vp, C.POINTER(C.c_long), C.POINTER(C.c_long), C.c_char_p, _sz]),
+    "glsm_batch_set_output": (_i, [_vp, _vp]),
+    "glsm_device_memory": (_i, [C.POINTER(_sz), C.POINTER(_sz)]),
     "glsm_system_step": (_i, [C.POINTER(_vp), _i, _vp, _vp, _i]),
     "glsm_copy_bandwidth": (_i, [_sz, _i, C.POINTER(_d)]),
 }
@@ -140,6 +150,11 @@ def _check(rc: int, what: str):
 
 def init(device: int = 0):
     _check(load().glsm_init(int(device)), "glsm_init")
+
+
+def set_interp_mode(mode: int):
+    """0 = by marker count, 1 = wavefront per marker, 2 = thread per marker."""
+    _check(load().glsm_set_interp_mode(int(mode)), "glsm_set_interp_mode")
 
 
 def synchronize():
@@ -397,9 +412,9 @@ class Batch:
             self.h = None
             self.members = {}
 
-    def add(self, gels, coupler, fluid, iteration=0, write_files=False):
-        mid = _check(load().glsm_batch_add(self.h, coupler.h, fluid.h, int(iteration), int(bool(write_files))),
-                     "batch add")
+    def add(self, gels, coupler, fluid, iteration=0, write_files=False, out_dir=""):
+        mid = _check(load().glsm_batch_add(self.h, coupler.h, fluid.h, int(iteration), int(bool(write_files)),
+                                           os.fsencode(out_dir)), "batch add")
         self.members[mid] = (gels, coupler, fluid)
         return mid
 
@@ -418,6 +433,12 @@ class Batch:
 
     def use_graph(self, on):
         _check(load().glsm_batch_use_graph(self.h, int(bool(on))), "batch use_graph")
+
+    def set_output(self, output):
+        """Route the members' .dat output through an OutputStream (None: the
+        objects' own synchronous writeFiles)."""
+        self.output = output
+        _check(load().glsm_batch_set_output(self.h, output.h if output else None), "batch set_output")
 
     def set_launch(self, nt, run, grid):
         _check(load().glsm_batch_set_launch(self.h, int(nt), int(run), int(grid)), "batch set_launch")
@@ -439,3 +460,44 @@ class Batch:
         ms, n = C.c_double(), C.c_int()
         _check(load().glsm_batch_lbm_timing(self.h, C.byref(ms), C.byref(n)), "batch lbm_timing")
         return ms.value, n.value
+
+
+class OutputStream:
+    """Asynchronous .dat output (include/output.h): device snapshot in stream
+    order, copy to pinned memory on a copy stream, formatting on writer
+    threads, in the reference's file layout."""
+
+    def __init__(self, writers=4):
+        h = C.c_void_p()
+        _check(load().glsm_output_create(int(writers), C.byref(h)), "glsm_output_create")
+        self.h = h.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().glsm_output_destroy(self.h)
+            self.h = None
+
+    def gel(self, gel, time, out_dir=""):
+        _check(load().glsm_output_gel(self.h, gel.h, int(time), os.fsencode(out_dir)), "output gel")
+
+    def fluid(self, fluid, time, out_dir=""):
+        _check(load().glsm_output_fluid(self.h, fluid.h, int(time), os.fsencode(out_dir)), "output fluid")
+
+    def release(self, obj):
+        _check(load().glsm_output_release(self.h, obj.h), "output release")
+
+    def flush(self):
+        _check(load().glsm_output_flush(self.h), "output flush")
+
+    def status(self):
+        p, c = C.c_long(), C.c_long()
+        err = C.create_string_buffer(512)
+        _check(load().glsm_output_status(self.h, C.byref(p), C.byref(c), err, 512), "output status")
+        return {"pending": p.value, "completed": c.value, "error": err.value.decode(errors="replace")}
+
+
+def device_memory():
+    """(free, total) bytes of the current device."""
+    f, t = C.c_size_t(), C.c_size_t()
+    _check(load().glsm_device_memory(C.byref(f), C.byref(t)), "glsm_device_memory")
+    return f.value, t.value

@@ -11,6 +11,9 @@
 // The members' iteration counters live in device memory, so the cadences
 // and the IBM ramp are evaluated on the device and the same launch sequence
 // serves every step: it is captured once per buffer parity in a HIP graph.
+// Marker positions change only on elasticity steps (iteration multiple of
+// 5) or from outside the batch, so the spreading lists are re-binned only on
+// steps where some member needs it (a second graph variant).
 #include "batch.h"
 
 #include <algorithm>
@@ -76,7 +79,8 @@ void Batch::dropGraphs() {
   graphs_.clear();
 }
 
-int Batch::add(std::vector<Gel*>& gels, Coupler* coupler, Fluid* fluid, int iter, bool write_files) {
+int Batch::add(std::vector<Gel*>& gels, Coupler* coupler, Fluid* fluid, int iter, bool write_files,
+               const std::string& out_dir) {
   if (!coupler || !fluid) throw std::invalid_argument("Batch::add: null coupler or fluid");
   if (&coupler->gels != &gels) throw std::invalid_argument("Batch::add: gels is not the coupler's vector");
   if (fluid->coupler != coupler) throw std::invalid_argument("Batch::add: fluid belongs to another coupler");
@@ -88,7 +92,7 @@ int Batch::add(std::vector<Gel*>& gels, Coupler* coupler, Fluid* fluid, int iter
   if (fluid->storage() != storage_ || (storage_ == Fluid::F16_DEV && fluid->storageScaleExp() != scale_exp_))
     fluid->setStorage(storage_, scale_exp_);
   GLSM_HIP(hipStreamSynchronize(glsm::stream()));
-  members_.push_back(Member{next_id_, &gels, coupler, fluid, iter, write_files});
+  members_.push_back(Member{next_id_, &gels, coupler, fluid, iter, write_files, out_dir});
   dirty_ = true;
   return next_id_++;
 }
@@ -196,7 +200,6 @@ void Batch::rebuild() {
   d.tables.M = mk;
   d.tables.ninst = (int)inst.size();
   // the stream/collide tuning copy writes u to scratch instead of the state
-  std::vector<float3> none;
   if (d.scratch_u.n < maxN) {
     glsm::dfree(d.scratch_u.p);
     d.scratch_u.p = glsm::dalloc<float3>(maxN);
@@ -213,15 +216,30 @@ void Batch::rebuild() {
   }
   GLSM_HIP(hipStreamSynchronize(s));
   dirty_ = false;
+  lists_fresh_ = false;
 }
 
-void Batch::enqueueStep(bool timed) {
+// Re-bin this step when the tables are new, some member's gels move in
+// this step's elasticity (iteration multiple of 5), or some gel moved
+// outside the batch since its last step.
+bool Batch::listsNeeded() const {
+  if (!lists_fresh_) return true;
+  size_t k = 0;
+  for (const Member& m : members_) {
+    if (m.iter % 5 == 0) return true;
+    for (Gel* g : *m.gels)
+      if (k >= seen_pos_.size() || g->positionVersion() != seen_pos_[k++]) return true;
+  }
+  return false;
+}
+
+void Batch::enqueueStep(bool timed, bool lists) {
   hipStream_t s = glsm::stream();
   glsm::BatchDevice& d = *dev_;
   const glsm::BatchTables& t = d.tables;
   glsm::launch_gel_step_batch(d.views.p, d.gel_inst.p, d.iters.p, d.ngel, d.max_L, s);
   glsm::launch_pack_batch(t, s);
-  glsm::launch_spread_rebuild_batch(t, s);
+  if (lists) glsm::launch_spread_rebuild_batch(t, s);
   const glsm::LbmLaunch shape = to_internal(launch_);
   for (int k = 0; k < nsub_; ++k) {
     glsm::launch_ibm_interp_batch(t, s);
@@ -252,14 +270,15 @@ void Batch::step(int nsteps) {
   hipStream_t s = glsm::stream();
   for (int n = 0; n < nsteps; ++n) {
     const bool timed = timing_every_ > 0 && (timing_calls_++ % timing_every_) == 0 && timing_events_.size() < 8192;
+    const bool lists = listsNeeded();
     if (use_graph_ && !timed) {
-      const int key = pf_ * 2 + pc_;
+      const int key = pf_ * 4 + pc_ * 2 + (lists ? 1 : 0);
       auto it = graphs_.find(key);
       if (it == graphs_.end()) {
         hipGraph_t g;
         GLSM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         try {
-          enqueueStep(false);
+          enqueueStep(false, lists);
         } catch (...) {
           (void)hipStreamEndCapture(s, &g);
           throw;
@@ -272,8 +291,9 @@ void Batch::step(int nsteps) {
       }
       GLSM_HIP(hipGraphLaunch((hipGraphExec_t)it->second, s));
     } else {
-      enqueueStep(timed);
+      enqueueStep(timed, lists);
     }
+    lists_fresh_ = true;
     // the objects follow the device state: buffers swapped as their own
     // step methods would have, positions marked as changed
     for (Member& m : members_) {
@@ -282,14 +302,23 @@ void Batch::step(int nsteps) {
       std::swap(f.d_c1, f.d_c2);
       for (Gel* g : *m.gels) g->markPositionsChanged();
     }
+    seen_pos_.clear();
+    for (Member& m : members_)
+      for (Gel* g : *m.gels) seen_pos_.push_back(g->positionVersion());
     if (nsub_ & 1) pf_ ^= 1;
     pc_ ^= 1;
     for (Member& m : members_) {
       const int it = m.iter++;
-      if (m.write_files) {
+      if (!m.write_files) continue;
+      if (!output_) {
         for (Gel* g : *m.gels) g->writeFiles(it);
         m.fluid->writeFiles(it);
+        continue;
       }
+      // the objects' cadence and output times (gel.cu:543-553, fluid.cu:140-150)
+      for (Gel* g : *m.gels)
+        if (it % 1000 == 0) output_->gel(g, int(it * g->m_dt), m.out_dir);
+      if (it % 2000 == 0) output_->fluid(m.fluid, int((double)(it * m.fluid->dt)), m.out_dir);
     }
   }
 }

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "batch.h"
+#include "output.h"
 #include "coupling.h"
 #include "fluid.h"
 #include "gel.h"
@@ -675,10 +676,11 @@ int glsm_batch_destroy(glsm_batch* b) {
     return GLSM_OK;
   });
 }
-int glsm_batch_add(glsm_batch* b, glsm_coupler* c, glsm_fluid* f, int iter, int write_files) {
+int glsm_batch_add(glsm_batch* b, glsm_coupler* c, glsm_fluid* f, int iter, int write_files,
+                   const char* out_dir) {
   return guarded([&] {
     glsm_coupler* cp = CP(c);
-    return B(b)->add(cp->gels, cp->c, F(f), iter, write_files != 0);
+    return B(b)->add(cp->gels, cp->c, F(f), iter, write_files != 0, out_dir ? out_dir : "");
   });
 }
 int glsm_batch_remove(glsm_batch* b, int id) {
@@ -737,6 +739,93 @@ int glsm_batch_lbm_timing(glsm_batch* b, double* mean_ms, int* samples) {
   return guarded([&] {
     REQUIRE(mean_ms && samples, "null output pointer");
     *mean_ms = B(b)->lbmKernelTiming(samples);
+    return GLSM_OK;
+  });
+}
+
+struct glsm_output {
+  OutputStream* o = nullptr;
+};
+namespace {
+OutputStream* O(glsm_output* o) {
+  REQUIRE(o && o->o, "null output handle");
+  return o->o;
+}
+}  // namespace
+int glsm_output_create(int writers, glsm_output** out) {
+  return guarded([&] {
+    REQUIRE(out, "null output pointer");
+    auto* h = new glsm_output;
+    try {
+      h->o = new OutputStream(writers);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+    return GLSM_OK;
+  });
+}
+int glsm_output_destroy(glsm_output* o) {
+  return guarded([&] {
+    delete O(o);
+    delete o;
+    return GLSM_OK;
+  });
+}
+int glsm_output_gel(glsm_output* o, glsm_gel* g, int time, const char* dir) {
+  return guarded([&] {
+    O(o)->gel(G(g), time, dir ? dir : "");
+    return GLSM_OK;
+  });
+}
+int glsm_output_fluid(glsm_output* o, glsm_fluid* f, int time, const char* dir) {
+  return guarded([&] {
+    O(o)->fluid(F(f), time, dir ? dir : "");
+    return GLSM_OK;
+  });
+}
+int glsm_output_release(glsm_output* o, const void* object) {
+  return guarded([&] {
+    O(o)->release(object);
+    return GLSM_OK;
+  });
+}
+int glsm_output_flush(glsm_output* o) {
+  return guarded([&] {
+    O(o)->flush();
+    return GLSM_OK;
+  });
+}
+int glsm_output_status(glsm_output* o, long* pending, long* completed, char* err, size_t err_len) {
+  return guarded([&] {
+    OutputStream* s = O(o);
+    if (pending) *pending = s->pending();
+    if (completed) *completed = s->completed();
+    if (err && err_len) {
+      const std::string e = s->lastError();
+      std::snprintf(err, err_len, "%s", e.c_str());
+    }
+    return GLSM_OK;
+  });
+}
+int glsm_batch_set_output(glsm_batch* b, glsm_output* o) {
+  return guarded([&] {
+    B(b)->setOutput(o ? O(o) : nullptr);
+    return GLSM_OK;
+  });
+}
+int glsm_device_memory(size_t* free_bytes, size_t* total_bytes) {
+  return guarded([&] {
+    REQUIRE(free_bytes && total_bytes, "null output pointer");
+    GLSM_HIP(hipMemGetInfo(free_bytes, total_bytes));
+    return GLSM_OK;
+  });
+}
+
+int glsm_set_interp_mode(int mode) {
+  return guarded([&] {
+    glsm::set_interp_mode(mode);
     return GLSM_OK;
   });
 }

@@ -127,9 +127,10 @@ struct SpreadLists {
   int* bsum;  // nblk + 1
   SpreadEntry* ent;
   // per marker, rebuilt with the lists (markers are fixed between rebuilds):
-  int* ctab;     // 64 per marker: stencil cell (x outer, z inner) or -1
-  float* wtab;   // 64 per marker: spreading weight phi * (1/h^3) / sum(phi)
-  float* rtab;   // 64 per marker: interpolation weight phi_x phi_y phi_z
+  // 64 entries per marker, lane-major (entry k of marker l at k * M + l):
+  int* ctab;     // stencil cell (x outer, z inner) or -1
+  float* wtab;   // spreading weight phi * (1/h^3) / sum(phi)
+  float* rtab;   // interpolation weight phi_x phi_y phi_z
   float* wsum;   // per marker: sum of rtab in stencil order
 };
 
@@ -139,6 +140,9 @@ void launch_spread_rebuild(const LatticeGeom& g, const MarkerArrays& m, const Sp
 // Ul = interp(u), Fdrag = beta_eff (Vl - Ul), from the rebuilt stencil tables
 void launch_ibm_interp(const MarkerArrays& m, const SpreadLists& sl, const float3* u, float beta_eff,
                        hipStream_t s);
+// 0 = choose by marker count, 1 = one wavefront per marker, 2 = one thread
+// per marker (identical results; a measurement and test switch)
+void set_interp_mode(int mode);
 // Dl = interp(c), Sl = Cl - Dl
 void launch_ibm_sample(const MarkerArrays& m, const SpreadLists& sl, const float* c, hipStream_t s);
 

@@ -173,9 +173,26 @@ __device__ __forceinline__ void repulsion_wave(const MarkerArrays& m, float delt
 // ---------------------------------------------------------------- IBM
 // Ul = sum(u*phi)/sum(phi) over the stencil, Fdrag = beta_eff*(Vl - Ul)
 // (k_ibm_interpolate_velocity + k_scale_negbeta, fluid_kernels.cu:244-291)
+// Stencil tables are stored lane-major: entry (marker l, stencil lane k) at
+// k * M + l, so one thread per marker reads them coalesced.
+__device__ __forceinline__ size_t tab(const MarkerArrays& m, int l, int k) { return (size_t)k * m.M + l; }
+
+// Ul / Fdrag from the 64 stencil sums (shared by both interpolation forms)
+__device__ __forceinline__ void interp_finish(const MarkerArrays& m, const SpreadLists& sl, float beta_eff, int l,
+                                              float ux, float uy, float uz) {
+  const float ws = sl.wsum[l];
+  float3 ul = make_float3(0.f, 0.f, 0.f);
+  if (ws > 0.0f) ul = make_float3(ux / ws, uy / ws, uz / ws);
+  m.Ul[l] = ul;
+  const float3 vl = m.Vl[l];
+  m.Fdrag[l] = make_float3(beta_eff * (vl.x - ul.x), beta_eff * (vl.y - ul.y), beta_eff * (vl.z - ul.z));
+}
+
+// One wavefront per marker, one lane per stencil cell, the 64 products
+// summed in lane order (for small marker counts: more parallelism).
 __device__ __forceinline__ void interp_wave(const MarkerArrays& m, const SpreadLists& sl,
                                             const float3* __restrict__ u, float beta_eff, int l, int lane) {
-  const size_t k = (size_t)l * 64 + lane;
+  const size_t k = tab(m, l, lane);
   const int cell = sl.ctab[k];
   float px = 0.f, py = 0.f, pz = 0.f;
   if (cell >= 0) {
@@ -192,32 +209,71 @@ __device__ __forceinline__ void interp_wave(const MarkerArrays& m, const SpreadL
     uy += lane_read(py, j);
     uz += lane_read(pz, j);
   }
-  if (lane == 0) {
-    const float ws = sl.wsum[l];
-    float3 ul = make_float3(0.f, 0.f, 0.f);
-    if (ws > 0.0f) ul = make_float3(ux / ws, uy / ws, uz / ws);
-    m.Ul[l] = ul;
-    const float3 vl = m.Vl[l];
-    m.Fdrag[l] = make_float3(beta_eff * (vl.x - ul.x), beta_eff * (vl.y - ul.y), beta_eff * (vl.z - ul.z));
+  if (lane == 0) interp_finish(m, sl, beta_eff, l, ux, uy, uz);
+}
+
+// One thread per (marker, velocity component) summing the 64 stencil cells
+// in order (for many markers: the serial sums run in all lanes at once).
+// The loads are unconditional -- a cell outside the lattice reads cell 0
+// with weight 0 -- so 32 iterations' loads are in flight together.  The
+// same products in the same order as the wavefront form, except that cells
+// outside the lattice add (+-0) instead of nothing: the running sum starts
+// at +0 and is never -0, so adding a zero leaves it unchanged and the two
+// forms agree bit for bit.
+__device__ __forceinline__ float stencil_sum(const MarkerArrays& m, const SpreadLists& sl,
+                                             const float* __restrict__ field, int stride, int comp, int l) {
+  constexpr int kChunk = 16;  // table entries, then field values, loaded ahead per chunk
+  float acc = 0.f;
+  for (int k0 = 0; k0 < 64; k0 += kChunk) {
+    int cell[kChunk];
+    float w[kChunk], f[kChunk];
+#pragma unroll
+    for (int j = 0; j < kChunk; ++j) {
+      const size_t t = tab(m, l, k0 + j);
+      cell[j] = max(sl.ctab[t], 0);
+      w[j] = sl.rtab[t];
+    }
+#pragma unroll
+    for (int j = 0; j < kChunk; ++j) f[j] = field[(size_t)stride * cell[j] + comp];
+#pragma unroll
+    for (int j = 0; j < kChunk; ++j) acc += f[j] * w[j];
   }
+  return acc;
+}
+
+__device__ __forceinline__ void interp_component(const MarkerArrays& m, const SpreadLists& sl,
+                                                 const float3* __restrict__ u, float beta_eff, int l, int comp) {
+  const float acc = stencil_sum(m, sl, reinterpret_cast<const float*>(u), 3, comp, l);
+  const float ws = sl.wsum[l];
+  const float ul = ws > 0.0f ? acc / ws : 0.f;
+  const float vl = reinterpret_cast<const float*>(m.Vl)[3 * (size_t)l + comp];
+  reinterpret_cast<float*>(m.Ul)[3 * (size_t)l + comp] = ul;
+  reinterpret_cast<float*>(m.Fdrag)[3 * (size_t)l + comp] = beta_eff * (vl - ul);
 }
 
 // Dl = sum(c*phi)/sum(phi), Sl = Cl - Dl (k_ibm_sample_concentration +
 // k_robin_boundary, fluid_kernels.cu:350-387, 293-298)
+__device__ __forceinline__ void sample_finish(const MarkerArrays& m, const SpreadLists& sl, int l, float cl) {
+  const float ws = sl.wsum[l];
+  const float dl = ws > 0.0f ? cl / ws : 0.0f;
+  m.Dl[l] = dl;
+  m.Sl[l] = 1 * (m.Cl[l] - dl);
+}
+
 __device__ __forceinline__ void sample_wave(const MarkerArrays& m, const SpreadLists& sl, const float* __restrict__ c,
                                             int l, int lane) {
-  const size_t k = (size_t)l * 64 + lane;
+  const size_t k = tab(m, l, lane);
   const int cell = sl.ctab[k];
   const float p = cell >= 0 ? c[cell] * sl.rtab[k] : 0.f;
   float cl = 0.f;
 #pragma unroll
   for (int j = 0; j < 64; ++j) cl += lane_read(p, j);
-  if (lane == 0) {
-    const float ws = sl.wsum[l];
-    const float dl = ws > 0.0f ? cl / ws : 0.0f;
-    m.Dl[l] = dl;
-    m.Sl[l] = 1 * (m.Cl[l] - dl);
-  }
+  if (lane == 0) sample_finish(m, sl, l, cl);
+}
+
+__device__ __forceinline__ void sample_thread(const MarkerArrays& m, const SpreadLists& sl,
+                                              const float* __restrict__ c, int l) {
+  sample_finish(m, sl, l, stencil_sum(m, sl, c, 1, 0, l));
 }
 
 // ---- spreading lists ----
@@ -232,7 +288,7 @@ __device__ __forceinline__ void spread_count_wave(const LatticeGeom& g, const Ma
   float ws = 0.f;
 #pragma unroll
   for (int j = 0; j < 64; ++j) ws += lane_read(st.w, j);
-  const size_t k = (size_t)l * 64 + lane;
+  const size_t k = tab(m, l, lane);
   if (lane == 0) sl.wsum[l] = ws;
   if (st.cell < 0 || ws == 0.0f) {
     sl.ctab[k] = -1;
@@ -305,12 +361,13 @@ __device__ __forceinline__ void local_block(const LatticeGeom& g, const SpreadLi
   }
 }
 
-__device__ __forceinline__ void fill_at(const SpreadLists& sl, int k) {
+// table entry k (of marker l)
+__device__ __forceinline__ void fill_at(const SpreadLists& sl, size_t k, int l) {
   const int cell = sl.ctab[k];
   if (cell < 0) return;
   const int pos = sl.rng[cell].x + atomicAdd(&sl.fillpos[cell], 1);
   SpreadEntry e;
-  e.marker = k >> 6;
+  e.marker = l;
   e.w = sl.wtab[k];
   sl.ent[pos] = e;
 }
@@ -417,6 +474,15 @@ __global__ void __launch_bounds__(kThreads) k_ibm_sample(MarkerArrays m, SpreadL
   const int l = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
   if (l < m.M) sample_wave(m, sl, c, l, threadIdx.x & 63);
 }
+__global__ void __launch_bounds__(kThreads) k_ibm_interp_t(MarkerArrays m, SpreadLists sl, const float3* __restrict__ u,
+                                                           float beta_eff) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < 3 * m.M) interp_component(m, sl, u, beta_eff, t / 3, t % 3);
+}
+__global__ void __launch_bounds__(kThreads) k_ibm_sample_t(MarkerArrays m, SpreadLists sl, const float* __restrict__ c) {
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l < m.M) sample_thread(m, sl, c, l);
+}
 __global__ void __launch_bounds__(kThreads) k_spread_count(LatticeGeom g, MarkerArrays m, SpreadLists sl) {
   const int l = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
   if (l < m.M) spread_count_wave(g, m, sl, l, threadIdx.x & 63);
@@ -432,9 +498,9 @@ __global__ void __launch_bounds__(kThreads) k_spread_scan_blocks(SpreadLists sl)
 __global__ void __launch_bounds__(kThreads) k_spread_local(LatticeGeom g, SpreadLists sl) {
   local_block(g, sl, blockIdx.x);
 }
-__global__ void __launch_bounds__(kThreads) k_spread_fill(int M64, SpreadLists sl) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < M64) fill_at(sl, k);
+__global__ void __launch_bounds__(kThreads) k_spread_fill(int M, SpreadLists sl) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;  // lane-major table entry
+  if (k < 64 * M) fill_at(sl, k, k % M);
 }
 __global__ void __launch_bounds__(kThreads) k_spread_sort(LatticeGeom g, SpreadLists sl) { sort_at(g, sl, blockIdx.x); }
 __global__ void __launch_bounds__(kThreads) k_scalar(LatticeGeom g, ScalarParams p, const float3* __restrict__ u,
@@ -490,6 +556,25 @@ __global__ void __launch_bounds__(kThreads) kb_ibm_interp(BatchTables t) {
   const BatchInst& I = marker_member(t, gm, l);
   interp_wave(I.m, I.sl, I.u, beta_eff_of(I, t.iters[t.mk_inst[gm]]), l, threadIdx.x & 63);
 }
+__global__ void __launch_bounds__(kThreads) kb_ibm_interp_t(BatchTables t) {
+  const int g3 = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g3 >= 3 * t.M) return;
+  const int gm = g3 / 3;
+  int l;
+  const BatchInst& I = marker_member(t, gm, l);
+  const MarkerArrays m = I.m;  // values in registers: the table loop unrolls
+  const SpreadLists sl = I.sl;
+  interp_component(m, sl, I.u, beta_eff_of(I, t.iters[t.mk_inst[gm]]), l, g3 % 3);
+}
+__global__ void __launch_bounds__(kThreads) kb_ibm_sample_t(BatchTables t, int cpar) {
+  const int gm = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gm >= t.M) return;
+  int l;
+  const BatchInst& I = marker_member(t, gm, l);
+  const MarkerArrays m = I.m;
+  const SpreadLists sl = I.sl;
+  sample_thread(m, sl, I.c[cpar], l);
+}
 __global__ void __launch_bounds__(kThreads) kb_ibm_sample(BatchTables t, int cpar) {
   const int gm = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
   if (gm >= t.M) return;
@@ -529,7 +614,7 @@ __global__ void __launch_bounds__(kThreads) kb_spread_fill(BatchTables t) {
   if (k >= 64L * t.M) return;
   int l;
   const BatchInst& I = marker_member(t, (int)(k >> 6), l);
-  fill_at(I.sl, l * 64 + (int)(k & 63));
+  fill_at(I.sl, tab(I.m, l, (int)(k & 63)), l);
 }
 __global__ void __launch_bounds__(kThreads) kb_spread_sort(BatchTables t) {
   int b;
@@ -559,15 +644,33 @@ void launch_repulsion(const MarkerArrays& m, float delta, float gamma, hipStream
   k_repulsion<<<blocks_for(m.M, kWavesPerBlock), kThreads, 0, s>>>(m, delta, gamma);
   GLSM_LAUNCH_CHECK();
 }
+// Interpolation form: 0 = by marker count, 1 = wavefront per marker,
+// 2 = thread per marker.  Both forms give the same bits.
+namespace {
+int g_interp_mode = 0;
+constexpr int kThreadFormMarkers = 2048;  // from this many markers on, one thread per marker
+bool thread_form(int M) { return g_interp_mode == 2 || (g_interp_mode == 0 && M >= kThreadFormMarkers); }
+}  // namespace
+void set_interp_mode(int mode) {
+  if (mode < 0 || mode > 2) throw std::invalid_argument("interpolation mode must be 0, 1 or 2");
+  g_interp_mode = mode;
+}
+
 void launch_ibm_interp(const MarkerArrays& m, const SpreadLists& sl, const float3* u, float beta_eff,
                        hipStream_t s) {
   if (m.M == 0) return;
-  k_ibm_interp<<<blocks_for(m.M, kWavesPerBlock), kThreads, 0, s>>>(m, sl, u, beta_eff);
+  if (thread_form(m.M))
+    k_ibm_interp_t<<<blocks_for(3L * m.M, kThreads), kThreads, 0, s>>>(m, sl, u, beta_eff);
+  else
+    k_ibm_interp<<<blocks_for(m.M, kWavesPerBlock), kThreads, 0, s>>>(m, sl, u, beta_eff);
   GLSM_LAUNCH_CHECK();
 }
 void launch_ibm_sample(const MarkerArrays& m, const SpreadLists& sl, const float* c, hipStream_t s) {
   if (m.M == 0) return;
-  k_ibm_sample<<<blocks_for(m.M, kWavesPerBlock), kThreads, 0, s>>>(m, sl, c);
+  if (thread_form(m.M))
+    k_ibm_sample_t<<<blocks_for(m.M, kThreads), kThreads, 0, s>>>(m, sl, c);
+  else
+    k_ibm_sample<<<blocks_for(m.M, kWavesPerBlock), kThreads, 0, s>>>(m, sl, c);
   GLSM_LAUNCH_CHECK();
 }
 
@@ -577,7 +680,7 @@ void launch_spread_rebuild(const LatticeGeom& g, const MarkerArrays& m, const Sp
   k_spread_block_sum<<<sl.nblk, kThreads, 0, s>>>(g, sl);
   k_spread_scan_blocks<<<1, kThreads, 0, s>>>(sl);
   k_spread_local<<<sl.nblk, kThreads, 0, s>>>(g, sl);
-  if (m.M > 0) k_spread_fill<<<blocks_for((long)m.M * 64, kThreads), kThreads, 0, s>>>(m.M * 64, sl);
+  if (m.M > 0) k_spread_fill<<<blocks_for((long)m.M * 64, kThreads), kThreads, 0, s>>>(m.M, sl);
   k_spread_sort<<<sl.nblk, kThreads, 0, s>>>(g, sl);
   GLSM_LAUNCH_CHECK();
 }
@@ -606,12 +709,18 @@ void launch_repulsion_batch(const BatchTables& t, hipStream_t s) {
 }
 void launch_ibm_interp_batch(const BatchTables& t, hipStream_t s) {
   if (t.M == 0) return;
-  kb_ibm_interp<<<blocks_for(t.M, kWavesPerBlock), kThreads, 0, s>>>(t);
+  if (thread_form(t.M))
+    kb_ibm_interp_t<<<blocks_for(3L * t.M, kThreads), kThreads, 0, s>>>(t);
+  else
+    kb_ibm_interp<<<blocks_for(t.M, kWavesPerBlock), kThreads, 0, s>>>(t);
   GLSM_LAUNCH_CHECK();
 }
 void launch_ibm_sample_batch(const BatchTables& t, int cpar, hipStream_t s) {
   if (t.M == 0) return;
-  kb_ibm_sample<<<blocks_for(t.M, kWavesPerBlock), kThreads, 0, s>>>(t, cpar);
+  if (thread_form(t.M))
+    kb_ibm_sample_t<<<blocks_for(t.M, kThreads), kThreads, 0, s>>>(t, cpar);
+  else
+    kb_ibm_sample<<<blocks_for(t.M, kWavesPerBlock), kThreads, 0, s>>>(t, cpar);
   GLSM_LAUNCH_CHECK();
 }
 void launch_spread_rebuild_batch(const BatchTables& t, hipStream_t s) {

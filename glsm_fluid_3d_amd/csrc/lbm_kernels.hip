@@ -56,14 +56,16 @@ __device__ __forceinline__ void put(T* p, T v) {
 }
 template <int S>
 struct Store;
-// st() stores v and returns the value kept, read back as ld() would.
+// Values move in stored units: f (full storage), d = f - w (deviations),
+// d * 2^scale_exp (FP16 deviations, `scaled`).  st() stores v and returns
+// the value kept, read back as ld() would.
 template <>
 struct Store<F32_FULL> {
   using T = float;
-  static constexpr bool dev = false;
-  __device__ static float ld(const T* p, float) { return *p; }
+  static constexpr bool dev = false, scaled = false;
+  __device__ static float ld(const T* p) { return *p; }
   template <bool NT>
-  __device__ static float st(T* p, float v, float, float) {
+  __device__ static float st(T* p, float v) {
     put<T, NT>(p, v);
     return v;
   }
@@ -71,10 +73,10 @@ struct Store<F32_FULL> {
 template <>
 struct Store<F32_DEV> {
   using T = float;
-  static constexpr bool dev = true;
-  __device__ static float ld(const T* p, float) { return *p; }
+  static constexpr bool dev = true, scaled = false;
+  __device__ static float ld(const T* p) { return *p; }
   template <bool NT>
-  __device__ static float st(T* p, float v, float, float) {
+  __device__ static float st(T* p, float v) {
     put<T, NT>(p, v);
     return v;
   }
@@ -82,28 +84,37 @@ struct Store<F32_DEV> {
 template <>
 struct Store<F16_DEV> {
   using T = _Float16;
-  static constexpr bool dev = true;
-  __device__ static float ld(const T* p, float inv_scale) { return (float)*p * inv_scale; }
+  static constexpr bool dev = true, scaled = true;
+  __device__ static float ld(const T* p) { return (float)*p; }
   template <bool NT>
-  __device__ static float st(T* p, float v, float scale, float inv_scale) {
-    const _Float16 h = (_Float16)(v * scale);
+  __device__ static float st(T* p, float v) {
+    const _Float16 h = (_Float16)v;
     put<T, NT>(p, h);
-    return (float)h * inv_scale;
+    return (float)h;
   }
 };
 template <>
 struct Store<BF16_DEV> {
   using T = uint16_t;
-  static constexpr bool dev = true;
-  __device__ static float ld(const T* p, float) { return __uint_as_float((uint32_t)*p << 16); }
+  static constexpr bool dev = true, scaled = false;
+  __device__ static float ld(const T* p) { return __uint_as_float((uint32_t)*p << 16); }
   template <bool NT>
-  __device__ static float st(T* p, float v, float, float) {
+  __device__ static float st(T* p, float v) {
     const uint32_t b = __float_as_uint(v);
     const uint32_t r = (b + 0x7FFFu + ((b >> 16) & 1u)) >> 16;  // round to nearest even (finite v)
     put<T, NT>(p, (uint16_t)r);
     return __uint_as_float(r << 16);
   }
 };
+// stored units <-> the population (or deviation) itself
+template <int S>
+__device__ __forceinline__ float to_value(float raw, const LbmParams& p) {
+  return Store<S>::scaled ? raw * p.inv_scale : raw;
+}
+template <int S>
+__device__ __forceinline__ float to_raw(float val, const LbmParams& p) {
+  return Store<S>::scaled ? val * p.scale : val;
+}
 
 __device__ inline void decode_cell(const LatticeGeom& g, int i, int& x, int& y, int& z) {
   const int sxy = g.Nx * g.Ny;
@@ -201,25 +212,29 @@ struct Unroll<0> {
 
 __device__ __forceinline__ float wq(const LbmParams& p, int q) { return q == 0 ? p.w0 : (q < 7 ? p.w1 : p.w2); }
 
-// BGK + Guo on one cell.  v[] holds f (full storage) or d = f - w (deviation
-// storage); the same representation is written back.
+// BGK + Guo on one cell.  v[] holds the stored units (f, d = f - w, or d
+// scaled by S = 2^scale_exp); the same units are written back.  The scale
+// enters per cell, not per population: rho = 1 + s/S, u = m / (rho S), the
+// equilibrium deviation w (s + rho S poly), the Guo term times S.
 // 16-bit storages: the moments are summed in FP64.  Every loaded value is a
 // 16-bit number times a power of two, and the 19 of a cell span far fewer
 // than 53 bits, so these sums are exact: rho and rho*u + F/2 are the exact
 // moments of the stored populations, each rounded once to FP32 for the
 // collision.  The collision conserves mass exactly (sum of the equilibria =
 // rho, the Guo term sums to 0), so the rest population is stored last as the
-// FP64 remainder of the cell's mass after the 18 moving ones were rounded:
-// a cell keeps its stored mass up to the one rounding of that remainder
-// instead of accumulating 19 roundings per sub-step.
+// remainder of the cell's mass after the 18 moving ones were rounded: a cell
+// keeps its stored mass up to the one rounding of that remainder instead of
+// accumulating 19 roundings per sub-step.
 template <int S, bool FORCED, bool NT>
 __device__ __forceinline__ void collide_store(const float (&v)[19], float Fx, float Fy, float Fz, const LbmParams& p,
                                               typename Store<S>::T* __restrict__ fout, unsigned plane, unsigned i,
                                               float3* u, bool store_u) {
   constexpr bool DEV = Store<S>::dev;
   constexpr bool MOM64 = S == F16_DEV || S == BF16_DEV;
-  float s, rho, mx, my, mz;  // mass deviation (DEV) or mass, density, momentum incl. F/2
-  double sd = 0.0;           // MOM64: exact mass deviation
+  constexpr bool SCALED = Store<S>::scaled;
+  const float sc = SCALED ? p.scale : 1.0f, isc = SCALED ? p.inv_scale : 1.0f;
+  float s, rho, mx, my, mz;  // (stored units) mass deviation or mass, momentum incl. F/2; rho itself
+  double sd = 0.0;           // MOM64: exact mass deviation (stored units)
   if (MOM64) {
 #pragma unroll
     for (int q = 0; q < 19; ++q) sd += (double)v[q];
@@ -230,12 +245,13 @@ __device__ __forceinline__ void collide_store(const float (&v)[19], float Fx, fl
     double mzd = ((double)v[5] - v[6]) + ((double)v[11] - v[14]) + ((double)v[12] - v[13]) +
                  ((double)v[15] - v[16]) + ((double)v[18] - v[17]);
     if (FORCED) {
-      mxd += 0.5 * (double)Fx;
-      myd += 0.5 * (double)Fy;
-      mzd += 0.5 * (double)Fz;
+      const double h = 0.5 * (double)sc;  // F/2 in stored units (exact)
+      mxd += h * (double)Fx;
+      myd += h * (double)Fy;
+      mzd += h * (double)Fz;
     }
     s = (float)sd;
-    rho = (float)(1.0 + sd);
+    rho = (float)(1.0 + sd * (double)isc);
     mx = (float)mxd;
     my = (float)myd;
     mz = (float)mzd;
@@ -253,7 +269,7 @@ __device__ __forceinline__ void collide_store(const float (&v)[19], float Fx, fl
       mz += 0.5f * Fz;
     }
   }
-  const float inv_rho = 1.0f / rho;
+  const float inv_rho = isc / rho;
   const float ux = mx * inv_rho, uy = my * inv_rho, uz = mz * inv_rho;
   if (store_u) u[i] = make_float3(ux, uy, uz);
   // The multiply-adds below are written as fmaf: each rounds once, the same
@@ -262,7 +278,8 @@ __device__ __forceinline__ void collide_store(const float (&v)[19], float Fx, fl
   const float uu15 = 1.5f * fmaf(ux, ux, fmaf(uy, uy, uz * uz));
   const float uF = FORCED ? fmaf(ux, Fx, fmaf(uy, Fy, uz * Fz)) : 0.f;
   const float kF = 1.0f - 0.5f * omega;
-  double kept = 0.0;  // MOM64: sum of the stored moving populations
+  const float rhos = SCALED ? rho * sc : rho;
+  float kept = 0.f;  // MOM64: sum of the stored moving populations
   Unroll<19>::run([&](auto qc) {
     constexpr int q = decltype(qc)::value;
     if (MOM64 && q == 0) return;  // stored last
@@ -272,17 +289,17 @@ __device__ __forceinline__ void collide_store(const float (&v)[19], float Fx, fl
     const float poly = fmaf(cu, fmaf(4.5f, cu, 3.0f), -uu15);
     // feq = w rho (1 + poly); deviation form: feq - w = w (s + rho poly);
     // out = v + omega (feq - v)
-    const float neq = DEV ? fmaf(w, fmaf(rho, poly, s), -v[q]) : fmaf(w * rho, 1.0f + poly, -v[q]);
+    const float neq = DEV ? fmaf(w, fmaf(rhos, poly, s), -v[q]) : fmaf(w * rho, 1.0f + poly, -v[q]);
     float out = fmaf(omega, neq, v[q]);
     if (FORCED) {
       // Guo: w (1 - omega/2) (3 (c.F - u.F) + 9 (c.u)(c.F))
       const float ciF = cx * Fx + cy * Fy + cz * Fz;
-      out = fmaf(w * kF, fmaf(9.0f * cu, ciF, 3.0f * (ciF - uF)), out);
+      out = fmaf(SCALED ? w * kF * sc : w * kF, fmaf(9.0f * cu, ciF, 3.0f * (ciF - uF)), out);
     }
-    const float st = Store<S>::template st<NT>(fout + (unsigned)q * plane + i, out, p.scale, p.inv_scale);
-    if (MOM64) kept += (double)st;
+    const float st = Store<S>::template st<NT>(fout + (unsigned)q * plane + i, out);
+    if (MOM64) kept += st;
   });
-  if (MOM64) Store<S>::template st<NT>(fout + i, (float)(sd - kept), p.scale, p.inv_scale);
+  if (MOM64) Store<S>::template st<NT>(fout + i, (float)(sd - (double)kept));
 }
 
 template <int S, bool NT>
@@ -300,7 +317,7 @@ __device__ __forceinline__ void lbm_block(const LatticeGeom& g, const LbmParams&
   float v[19];
   Unroll<19>::run([&](auto qc) {
     constexpr int q = decltype(qc)::value;
-    v[q] = Store<S>::ld(fin + a.src<q>(i, plane), p.inv_scale);
+    v[q] = Store<S>::ld(fin + a.src<q>(i, plane));
   });
   const bool store_u = write_u_all || flagged;
   if (flagged || const_force) {
@@ -322,8 +339,9 @@ __device__ __forceinline__ void lbm_block(const LatticeGeom& g, const LbmParams&
   }
 }
 
+// 8 waves per SIMD (<= 64 VGPRs): keeps enough loads in flight per CU
 template <int S, bool NT>
-__global__ void __launch_bounds__(kThreads) k_lbm_step(LatticeGeom g, LbmParams p,
+__global__ void __launch_bounds__(kThreads, 8) k_lbm_step(LatticeGeom g, LbmParams p,
                                                        const typename Store<S>::T* __restrict__ fin,
                                                        typename Store<S>::T* __restrict__ fout, unsigned plane,
                                                        float3* __restrict__ u, int write_u_all, SpreadLists sl,
@@ -337,7 +355,7 @@ __global__ void __launch_bounds__(kThreads) k_lbm_step(LatticeGeom g, LbmParams 
 // stride gridDim.x (persistent when the grid is smaller than the block
 // count), the XCD remap applied to the flattened index.
 template <int S, bool NT, int RUN>
-__global__ void __launch_bounds__(kThreads) kb_lbm_step(BatchTables t, int sel, int write_u_all) {
+__global__ void __launch_bounds__(kThreads, 8) kb_lbm_step(BatchTables t, int sel, int write_u_all) {
   using T = typename Store<S>::T;
   for (unsigned w = blockIdx.x; w < (unsigned)t.nblk; w += gridDim.x) {
     const unsigned gb = xcd_block<RUN>(w, (unsigned)t.nblk);
@@ -363,7 +381,7 @@ __global__ void __launch_bounds__(kThreads) k_lbm_init(LatticeGeom g, LbmParams 
   if (i >= (unsigned)g.N) return;
 #pragma unroll
   for (int q = 0; q < 19; ++q)
-    Store<S>::template st<false>(f + (unsigned)q * plane + i, Store<S>::dev ? 0.f : wq(p, q), p.scale, p.inv_scale);
+    Store<S>::template st<false>(f + (unsigned)q * plane + i, Store<S>::dev ? 0.f : wq(p, q));
   u[i] = make_float3(0.f, 0.f, 0.f);
   c1[i] = 0.f;
   c2[i] = 0.f;
@@ -383,7 +401,7 @@ __global__ void __launch_bounds__(kThreads) k_lbm_export(LatticeGeom g, LbmParam
   Unroll<19>::run([&](auto qc) {
     constexpr int q = decltype(qc)::value;
     const unsigned s = PULL ? a.src<q>(i, plane) : (unsigned)q * plane + i;
-    const float v = Store<S>::ld(fpost + s, p.inv_scale);
+    const float v = to_value<S>(Store<S>::ld(fpost + s), p);
     out[(unsigned)q * out_plane + i] = Store<S>::dev ? wq(p, q) + v : v;
   });
 }
@@ -402,7 +420,7 @@ __global__ void __launch_bounds__(kThreads) k_lbm_import(LatticeGeom g, LbmParam
   Unroll<19>::run([&](auto qc) {
     constexpr int q = decltype(qc)::value;
     const float v = in[a.dst<q>(i, in_plane)];  // from plane q or opp q; w_opp(q) == w_q
-    Store<S>::template st<false>(fpost + (unsigned)q * plane + i, Store<S>::dev ? v - wq(p, q) : v, p.scale, p.inv_scale);
+    Store<S>::template st<false>(fpost + (unsigned)q * plane + i, to_raw<S>(Store<S>::dev ? v - wq(p, q) : v, p));
   });
 }
 
@@ -450,7 +468,7 @@ __global__ void __launch_bounds__(kThreads) k_lbm_moments(LatticeGeom g, LbmPara
     double m = Store<S>::dev ? 0.0 : -1.0;
     Unroll<19>::run([&](auto qc) {
       constexpr int q = decltype(qc)::value;
-      const double f = (double)Store<S>::ld(fpost + a.src<q>(i, plane), p.inv_scale);
+      const double f = (double)to_value<S>(Store<S>::ld(fpost + a.src<q>(i, plane)), p);
       // deviation storage: sum d = rho - 1 (the weights sum to 1), and
       // sum d c = sum f c (the weights of opposite velocities are equal)
       m += f;

@@ -18,6 +18,7 @@
 #include <stdexcept>
 #include <string>
 #include <thread>
+#include <vector>
 
 #include "fluid.h"
 #include "gel.h"
@@ -58,7 +59,7 @@ struct OutputState {
   long pending = 0, done = 0;
   std::string err;
   bool stop = false;
-  std::thread writer;
+  std::vector<std::thread> writers;
 };
 
 namespace {
@@ -182,9 +183,10 @@ void d2d(char*& dst, const void* src, size_t bytes) {
 }  // namespace
 }  // namespace glsm
 
-OutputStream::OutputStream() : st_(new glsm::OutputState) {
+OutputStream::OutputStream(int writers) : st_(new glsm::OutputState) {
+  if (writers < 1 || writers > 64) throw std::invalid_argument("OutputStream: writers must be in [1, 64]");
   GLSM_HIP(hipStreamCreate(&st_->copy));  // default priority
-  st_->writer = std::thread(glsm::writer_loop, st_.get());
+  for (int i = 0; i < writers; ++i) st_->writers.emplace_back(glsm::writer_loop, st_.get());
 }
 
 OutputStream::~OutputStream() {
@@ -193,7 +195,8 @@ OutputStream::~OutputStream() {
     st_->stop = true;
     st_->cv.notify_all();
   }
-  if (st_->writer.joinable()) st_->writer.join();
+  for (auto& w : st_->writers)
+    if (w.joinable()) w.join();
   (void)hipStreamSynchronize(st_->copy);
   st_->slots.clear();
   (void)hipStreamDestroy(st_->copy);

@@ -1,0 +1,270 @@
+"""Serving: a request loop with continuous batching over one device.
+
+Jobs (lattice, gels, rate constants, horizon) arrive over time.  The server
+  * queues them in arrival order;
+  * admits the head of the queue once its device footprint fits the HBM
+    budget and the active lattice cells stay under `max_active_cells` (the
+    size at which the device is saturated: more only stretches every job);
+  * steps every admitted job together through one Batch (one launch per
+    phase, HIP-graph captured; jobs join and leave between steps, each
+    giving the same bits as when run alone);
+  * streams each job's .dat files (the reference's layout and cadence)
+    through an asynchronous OutputStream into its own directory;
+  * retires a job the step its horizon is reached and records its arrival,
+    admission and completion times.
+The loop runs on the host between chunks of steps; a chunk never passes the
+nearest horizon, so completions are exact to the step.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import time
+from collections import deque
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import workloads as W
+
+NSUB = 25  # LBM sub-steps per time step (fluid.cu:194-197 at the reference constants)
+STORAGE_BYTES = {"f32full": 4, "f32": 4, "f16": 2, "bf16": 2}
+_ALLOC_SLACK = 64 << 10  # per device allocation: allocator granularity
+
+
+def gel_bytes(n: int) -> int:
+    """Device bytes of one n^3-element gel (Gel's allocations, gel.cpp)."""
+    L = n + 1
+    nE, nN = (L + 1) ** 3, (L + 2) ** 3
+    M = L ** 3 - (L - 2) ** 3
+    b = 10 * nE * 8 + nN * 8          # um, um_norm, vm_norm, wm, wmp, T0, T1, T2, vol, pm; vm (node-sized)
+    b += 3 * nN * 8                   # un_norm, un_robin, vn_norm
+    b += 6 * nN * 24                  # rn, Fn, Fn_robin, Fdrag_robin, Veln, Vels
+    b += (1 + 8 + 6) * nE * 24        # rm, rm_loc, nmSm
+    b += 4 * (nN + nE + M)            # maps, boundary index
+    b += 2 * 1000 * 8 + 3 * 1000 * 24 + 1000 * 300 * 24  # centre ring, filament
+    return b + 40 * _ALLOC_SLACK
+
+
+def fluid_bytes(edge: int, markers: int, storage: str) -> int:
+    """Device bytes of a Fluid of edge^3 cells with its Coupler (fluid.cpp,
+    coupling.cpp)."""
+    N = edge ** 3
+    plane = (N + 63) // 64 * 64
+    nblk = (N + 255) // 256
+    M = markers
+    b = 2 * 19 * plane * STORAGE_BYTES[storage]   # ping-pong distributions
+    b += N * (12 + 4 + 4 + 4 + 4 + 8)             # u, c1, c2, counts, fill cursors, ranges
+    b += nblk * (4 + 4 + 32) + 64                 # flags, block sums, FP64 partials
+    b += 64 * M * (8 + 12) + 4 * M                # entries, stencil tables, weight sums
+    b += M * (5 * 12 + 3 * 4 + 2 * 4)             # coupler marker arrays
+    return b + 30 * _ALLOC_SLACK
+
+
+def output_bytes(edge: int, gel_sizes) -> int:
+    """Staging buffers of the asynchronous output (device + pinned host)."""
+    b = 16 * edge ** 3
+    for n in gel_sizes:
+        L = n + 1
+        b += (9 * (L + 2) ** 3 + 6 * (L + 1) ** 3 + 11000) * 8
+    return b
+
+
+def job_footprint(job: W.Instance, storage: str, outputs: bool) -> int:
+    sizes = [g[0] for g in job.gels]
+    M = sum((n + 1) ** 3 - (n - 1) ** 3 for n in sizes)
+    b = fluid_bytes(job.edge, M, storage) + sum(gel_bytes(n) for n in sizes)
+    if outputs:
+        b += output_bytes(job.edge, sizes)
+    return b
+
+
+@dataclass
+class JobRecord:
+    job: W.Job
+    footprint: int
+    admitted: float = None
+    completed: float = None
+    member: int = None
+    objects: tuple = None
+    out_dir: str = ""
+    steps: int = 0
+    extra: dict = field(default_factory=dict)
+
+
+class Server:
+    """Continuous-batching request loop (see the module docstring).
+
+    mod: the library module (glsm_fluid_3d_amd).  hbm_budget: bytes the jobs
+    may occupy (default 80% of the free device memory at start).
+    out_root: directory for the jobs' .dat output (None = no output).
+    keep_output: keep each finished job's directory (else removed once its
+    files are written).  on_complete(record, gels, coupler, fluid) is called
+    before a finished job's objects are released."""
+
+    def __init__(self, mod, storage="f16", hbm_budget=None, max_active_cells=16_000_000, max_active_jobs=256,
+                 out_root=None, keep_output=True, writers=4, max_chunk=50, autotune=True, on_complete=None):
+        self.G = mod
+        self.storage = storage
+        free, total = mod.device_memory()
+        self.hbm_budget = int(hbm_budget if hbm_budget is not None else 0.8 * free)
+        self.max_active_cells = max_active_cells
+        self.max_active_jobs = max_active_jobs
+        self.out_root = out_root
+        self.keep_output = keep_output
+        self.max_chunk = max_chunk
+        self.autotune = autotune
+        self.on_complete = on_complete
+        self.batch = mod.Batch(storage)
+        self.output = mod.OutputStream(writers) if out_root else None
+        if self.output:
+            self.batch.set_output(self.output)
+        self.queue = deque()
+        self.active = {}
+        self.done = []
+        self.used_bytes = 0
+        self.active_cells = 0
+        self.cell_updates = 0
+        self.steps = 0
+        self.tuned = False
+
+    # ---- admission ---------------------------------------------------------
+    def submit(self, job: W.Job):
+        self.queue.append(JobRecord(job, job_footprint(job, self.storage, self.out_root is not None)))
+
+    def _fits(self, rec: JobRecord) -> bool:
+        if len(self.active) >= self.max_active_jobs:
+            return False
+        if self.used_bytes + rec.footprint > self.hbm_budget:
+            return False
+        return not self.active or self.active_cells + rec.job.cells <= self.max_active_cells
+
+    def _admit(self, now: float) -> int:
+        n = 0
+        while self.queue and self._fits(self.queue[0]):
+            rec = self.queue.popleft()
+            objs = W.build(self.G, rec.job, self.storage)
+            if self.out_root:
+                rec.out_dir = os.path.join(self.out_root, f"job{rec.job.job_id}")
+                os.makedirs(rec.out_dir, exist_ok=True)
+            rec.member = self.batch.add(*objs, iteration=0, write_files=self.out_root is not None,
+                                        out_dir=rec.out_dir)
+            rec.objects = objs
+            rec.admitted = now
+            self.active[rec.member] = rec
+            self.used_bytes += rec.footprint
+            self.active_cells += rec.job.cells
+            n += 1
+        if n and self.autotune and not self.tuned and self.active_cells >= self.max_active_cells // 2:
+            self.batch.autotune(reps=2)
+            self.tuned = True
+        return n
+
+    # ---- completion --------------------------------------------------------
+    def _retire(self, now: float):
+        finished = [r for r in self.active.values() if self.batch.iteration(r.member) >= r.job.horizon]
+        for rec in finished:
+            self.batch.remove(rec.member)
+            gels, c, f = rec.objects
+            if self.output:
+                for o in gels + [f]:
+                    self.output.release(o)
+            if self.on_complete:
+                self.on_complete(rec, gels, c, f)
+            for o in gels + [f, c]:
+                o.close()
+            rec.objects = None
+            rec.completed = now
+            rec.steps = rec.job.horizon
+            del self.active[rec.member]
+            self.used_bytes -= rec.footprint
+            self.active_cells -= rec.job.cells
+            self.done.append(rec)
+            if self.out_root and not self.keep_output:
+                shutil.rmtree(rec.out_dir, ignore_errors=True)
+
+    # ---- the loop ----------------------------------------------------------
+    def run(self, jobs, duration=None, clock=None, idle_sleep=0.002):
+        """Serve `jobs` (arrival times relative to the start) until all are
+        done or `duration` seconds have passed.  `clock` () -> seconds since
+        start (default: wall time)."""
+        t0 = time.perf_counter()
+        now_fn = clock or (lambda: time.perf_counter() - t0)
+        pending = deque(sorted(jobs, key=lambda j: j.arrival))
+        self.t_start = now_fn()
+        while True:
+            now = now_fn()
+            if duration is not None and now >= duration:
+                break
+            while pending and pending[0].arrival <= now:
+                self.submit(pending.popleft())
+            self._admit(now)
+            if not self.active:
+                if not pending and not self.queue:
+                    break
+                if clock is None:
+                    time.sleep(idle_sleep)
+                else:
+                    clock.advance_to(pending[0].arrival if pending else now)
+                continue
+            left = min(r.job.horizon - self.batch.iteration(r.member) for r in self.active.values())
+            n = max(1, min(self.max_chunk, left))
+            self.batch.step(n)
+            self.G.synchronize()
+            self.steps += n
+            self.cell_updates += self.active_cells * NSUB * n
+            if clock is not None:
+                clock.advance(n)
+            self._retire(now_fn())
+        self.elapsed = now_fn() - self.t_start
+        if self.output:
+            self.output.flush()
+        return self.stats()
+
+    def stats(self):
+        lat = np.array([r.completed - r.job.arrival for r in self.done]) if self.done else np.zeros(0)
+        wait = np.array([r.admitted - r.job.arrival for r in self.done]) if self.done else np.zeros(0)
+        pct = lambda a, q: float(np.percentile(a, q)) if a.size else None
+        out = {
+            "completed": len(self.done), "active": len(self.active), "queued": len(self.queue),
+            "latency_p50_s": pct(lat, 50), "latency_p99_s": pct(lat, 99),
+            "queue_wait_p50_s": pct(wait, 50), "queue_wait_p99_s": pct(wait, 99),
+            "mlups": self.cell_updates / max(self.elapsed, 1e-9) / 1e6 if hasattr(self, "elapsed") else None,
+            "elapsed_s": getattr(self, "elapsed", None), "batch_steps": self.steps,
+            "hbm_budget_GB": self.hbm_budget / 1e9,
+        }
+        if self.output:
+            out["output"] = self.output.status()
+        return out
+
+    def close(self):
+        for rec in list(self.active.values()):
+            self.batch.remove(rec.member)
+            gels, c, f = rec.objects
+            if self.output:
+                for o in gels + [f]:
+                    self.output.release(o)
+            for o in gels + [f, c]:
+                o.close()
+        self.active.clear()
+        if self.output:
+            self.output.close()
+        self.batch.close()
+
+
+class StepClock:
+    """A virtual clock for tests: time advances by `dt` per batch step, or
+    jumps to the next arrival when the server is idle."""
+
+    def __init__(self, dt=1.0):
+        self.t = 0.0
+        self.dt = dt
+
+    def __call__(self):
+        return self.t
+
+    def advance(self, steps):
+        self.t += steps * self.dt
+
+    def advance_to(self, t):
+        self.t = max(self.t, t)

@@ -1,0 +1,125 @@
+"""Seeded synthetic workloads: instance and job descriptions (no device work).
+
+Lattice sizes follow the reference's grid rule (fluid.cu:167-197): a
+physical box of side b gives int(b / h) + 1 nodes with h = 0.5, so an odd
+lattice edge E is the box (E - 1) / 2.  Gels are cubes of n elements (dx =
+1, gel.cu:633-652) centred at the given position.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+F_REF, EP_REF = 0.9, 0.3  # gel chemistry rate constants (gel.cu:633-652)
+
+
+def round_to_odd(x: float) -> int:
+    return int(2 * round((x - 1) / 2) + 1)
+
+
+def place_gels(rng, box: float, sizes, margin: float = 2.0, tries: int = 200):
+    """Seeded centres for cube gels of the given element counts inside a
+    cubic box: at least `margin` from the walls and one element apart from
+    each other (their boxes separated along some axis).  A gel that finds no
+    place in `tries` draws is dropped.  Returns [(centre, n)]."""
+    out = []
+    for n in sizes:
+        half = n / 2.0
+        lo, hi = half + margin, box - half - margin
+        if hi <= lo:
+            continue
+        for _ in range(tries):
+            p = rng.uniform(lo, hi, 3)
+            if all(np.any(np.abs(p - q) >= half + m / 2.0 + 1.0) for q, m in out):
+                out.append((p, n))
+                break
+    return out
+
+
+@dataclass
+class Instance:
+    """One gel-fluid instance: cubic lattice of `edge` nodes, gels as
+    (n elements, centre, f, ep), perturbation seed."""
+    edge: int
+    gels: list
+    seed: int
+
+    @property
+    def box(self) -> float:
+        return (self.edge - 1) * 0.5
+
+    @property
+    def cells(self) -> int:
+        return self.edge ** 3
+
+
+def batch_instances(count=64, seed=3, edge_range=(33, 97), gels_range=(1, 4), n_range=(4, 8), rate_jitter=0.2):
+    """The batching workload: lattice edge log-uniform in edge_range rounded
+    to odd, 1-4 cube gels of 4^3-8^3 elements at seeded non-overlapping
+    positions, rate constants f, ep uniform within +-20% of the reference's."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        edge = round_to_odd(np.exp(rng.uniform(np.log(edge_range[0]), np.log(edge_range[1]))))
+        box = (edge - 1) * 0.5
+        ng = int(rng.integers(gels_range[0], gels_range[1] + 1))
+        sizes = [int(rng.integers(n_range[0], n_range[1] + 1)) for _ in range(ng)]
+        placed = place_gels(rng, box, sizes)
+        gels = [(n, tuple(float(v) for v in p), F_REF * rng.uniform(1 - rate_jitter, 1 + rate_jitter),
+                 EP_REF * rng.uniform(1 - rate_jitter, 1 + rate_jitter)) for p, n in placed]
+        out.append(Instance(edge, gels, seed * 1000 + k))
+    return out
+
+
+@dataclass
+class Job(Instance):
+    """A simulation request: an instance, a horizon in time steps and its
+    arrival time (seconds from the start of the stream)."""
+    job_id: int = 0
+    horizon: int = 0
+    arrival: float = 0.0
+    extra: dict = field(default_factory=dict)
+
+
+def poisson_jobs(seed=4, rate=20.0, duration=600.0, edge_range=(41, 161), gels_range=(1, 6), n=6,
+                 horizon_range=(1000, 20000), max_jobs=None):
+    """The serving workload: Poisson arrivals (mean `rate` jobs/s) over
+    `duration` seconds; lattice edge log-uniform in edge_range rounded to
+    odd, 1-6 gels of n^3 elements at seeded non-overlapping positions with
+    the reference's rate constants, horizon uniform in horizon_range."""
+    rng = np.random.default_rng(seed)
+    jobs = []
+    t = 0.0
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t > duration or (max_jobs is not None and len(jobs) >= max_jobs):
+            break
+        edge = round_to_odd(np.exp(rng.uniform(np.log(edge_range[0]), np.log(edge_range[1]))))
+        box = (edge - 1) * 0.5
+        ng = int(rng.integers(gels_range[0], gels_range[1] + 1))
+        placed = place_gels(rng, box, [n] * ng)
+        gels = [(m, tuple(float(v) for v in p), F_REF, EP_REF) for p, m in placed]
+        horizon = int(rng.integers(horizon_range[0], horizon_range[1] + 1))
+        k = len(jobs)
+        jobs.append(Job(edge=edge, gels=gels, seed=seed * 100000 + k, job_id=k + 1, horizon=horizon, arrival=t))
+    return jobs
+
+
+def build(mod, inst: Instance, storage=None):
+    """The instance's objects through `mod` (the library or the oracle):
+    gels (rates set, chemistry perturbed by 0.01 from the seed), coupler,
+    fluid (walls)."""
+    gels = []
+    for j, (n, pos, f, ep) in enumerate(inst.gels):
+        g = mod.Gel((n, n, n), pos, gel_id=j + 1)
+        if (f, ep) != (F_REF, EP_REF):
+            g.set_chemistry_rates(f, ep)
+        g.perturb_chemistry(inst.seed * 8 + j, 0.01)
+        gels.append(g)
+    c = mod.Coupler(gels)
+    b = int(round(inst.box))
+    f = mod.Fluid((b, b, b), c)
+    if storage:
+        f.set_storage(storage)
+    return gels, c, f

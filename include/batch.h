@@ -21,6 +21,7 @@
 #include "coupling.h"
 #include "fluid.h"
 #include "gel.h"
+#include "output.h"
 
 namespace glsm {
 struct BatchDevice;
@@ -40,7 +41,10 @@ class Batch {
   // of the instance's next step.  With write_files the .dat output of
   // Gel::writeFiles / Fluid::writeFiles is produced at the reference's
   // cadence after each step.  Returns the member id.
-  int add(std::vector<Gel*>& gels, Coupler* coupler, Fluid* fluid, int iter, bool write_files = false);
+  // With an output stream attached (setOutput) the files go through it,
+  // into `out_dir`; without one, through the objects' own writeFiles.
+  int add(std::vector<Gel*>& gels, Coupler* coupler, Fluid* fluid, int iter, bool write_files = false,
+          const std::string& out_dir = "");
   void remove(int id);
   int size() const { return (int)members_.size(); }
   std::vector<int> ids() const;
@@ -52,6 +56,10 @@ class Batch {
 
   // Capture each step in a HIP graph (default on).
   void useGraph(bool on);
+
+  // Asynchronous .dat output for members added with write_files (nullptr =
+  // the objects' synchronous writeFiles).  The stream must outlive its use.
+  void setOutput(OutputStream* out) { output_ = out; }
 
   // Launch shape of the batched stream/collide kernel.
   struct Launch {
@@ -78,9 +86,11 @@ class Batch {
     Fluid* fluid;
     int iter;
     bool write_files;
+    std::string out_dir;
   };
   void rebuild();
-  void enqueueStep(bool timed);
+  bool listsNeeded() const;
+  void enqueueStep(bool timed, bool lists);
   void dropGraphs();
 
   int storage_;
@@ -95,8 +105,11 @@ class Batch {
   int nsub_ = 0;
   Launch launch_{0, 0, 0};
   std::unique_ptr<glsm::BatchDevice> dev_;
-  std::map<int, void*> graphs_;  // (pf, pc) -> hipGraphExec_t
+  bool lists_fresh_ = false;        // IBM lists match the members' marker positions
+  std::vector<uint64_t> seen_pos_;  // gels' position versions after the last step
+  std::map<int, void*> graphs_;     // (pf, pc, lists) -> hipGraphExec_t
   int timing_every_ = 0;
   long timing_calls_ = 0;
   std::vector<void*> timing_events_;
+  OutputStream* output_ = nullptr;
 };

@@ -186,6 +186,12 @@ int glsm_fluid_write(glsm_fluid* f, int field, const void* host, size_t bytes);
    roofline and for calibrating HBM traffic counters. */
 int glsm_copy_bandwidth(size_t bytes, int reps, double* gbps);
 
+/* Extension (measurement and tests): form of the IBM interpolation and
+   concentration sampling kernels, 0 = by marker count (default), 1 = one
+   wavefront per marker, 2 = one thread per marker.  Every form sums the 64
+   stencil products in the same order, so results are identical. */
+int glsm_set_interp_mode(int mode);
+
 /* ---- Batch (include/batch.h; no reference counterpart): many independent
    instances, each a coupler's gels + the coupler + its fluid, advanced one
    time step at a time by one launch per phase over all of them (HIP-graph
@@ -196,9 +202,11 @@ typedef struct glsm_batch glsm_batch;
 int glsm_batch_create(int storage, int scale_exp, glsm_batch** out);
 int glsm_batch_destroy(glsm_batch* b);
 /* iter = the instance's next iteration; write_files = produce the .dat
-   output (Gel::writeFiles / Fluid::writeFiles cadence) after each step.
-   Returns the member id (> 0) or a negative status. */
-int glsm_batch_add(glsm_batch* b, glsm_coupler* c, glsm_fluid* f, int iter, int write_files);
+   output (Gel::writeFiles / Fluid::writeFiles cadence) after each step,
+   into out_dir when an output stream is attached (NULL or "" = the working
+   directory).  Returns the member id (> 0) or a negative status. */
+int glsm_batch_add(glsm_batch* b, glsm_coupler* c, glsm_fluid* f, int iter, int write_files,
+                   const char* out_dir);
 int glsm_batch_remove(glsm_batch* b, int id);
 int glsm_batch_size(glsm_batch* b);
 int glsm_batch_iteration(glsm_batch* b, int id);
@@ -214,6 +222,26 @@ int glsm_batch_get_launch(glsm_batch* b, int* nt, int* run, int* grid);
 int glsm_batch_autotune(glsm_batch* b, int reps, double* best_ms);
 int glsm_batch_enable_lbm_timing(glsm_batch* b, int every);
 int glsm_batch_lbm_timing(glsm_batch* b, double* mean_ms, int* samples);
+
+/* ---- asynchronous .dat output (include/output.h; no reference
+   counterpart: the reference's writeFiles copies synchronously).  A request
+   snapshots the fields device-to-device in stream order, copies them to
+   pinned host memory on a copy stream and formats the reference's files on
+   writer threads while stepping goes on. ---- */
+typedef struct glsm_output glsm_output;
+int glsm_output_create(int writers, glsm_output** out);
+int glsm_output_destroy(glsm_output* o);       /* waits for pending writes */
+int glsm_output_gel(glsm_output* o, glsm_gel* g, int time, const char* dir);
+int glsm_output_fluid(glsm_output* o, glsm_fluid* f, int time, const char* dir);
+int glsm_output_release(glsm_output* o, const void* object);  /* a gel or fluid handle */
+int glsm_output_flush(glsm_output* o);
+/* requests not yet written / written; the last write error ("" if none) */
+int glsm_output_status(glsm_output* o, long* pending, long* completed, char* err, size_t err_len);
+/* attach (or detach with NULL) the stream to a batch */
+int glsm_batch_set_output(glsm_batch* b, glsm_output* o);
+
+/* device memory of the current device, bytes (hipMemGetInfo) */
+int glsm_device_memory(size_t* free_bytes, size_t* total_bytes);
 
 /* ---- one time step in sim.cpp's order (sim.cpp:66-81): for each gel
    stepElasticity, stepChemistry, recordCenter; packFromGels; stepVelocity;

@@ -23,7 +23,9 @@ struct OutputState;
 
 class OutputStream {
  public:
-  OutputStream();
+  // `writers` threads format files in parallel (requests for one object
+  // stay in order: each waits for the previous one of that object)
+  explicit OutputStream(int writers = 4);
   ~OutputStream();  // waits for every pending write
   OutputStream(const OutputStream&) = delete;
   OutputStream& operator=(const OutputStream&) = delete;

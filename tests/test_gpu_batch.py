@@ -178,3 +178,28 @@ def test_batch_rejects_bad_members(glsm):
     finally:
         b.close()
         close(inst)
+
+
+def test_interpolation_forms_identical(glsm):
+    """One wavefront per marker (solo runs) and one thread per marker (batch)
+    sum the same 64 products in the same order: identical bits."""
+    G = glsm
+    try:
+        G.set_interp_mode(1)
+        ref = [solo(G, s, 0, 8) for s in SPECS]
+        G.set_interp_mode(2)
+        b = G.Batch("f32")
+        insts = [build(G, s) for s in SPECS]
+        try:
+            for inst in insts:
+                b.add(*inst)
+            b.step(8)
+            for k, inst in enumerate(insts):
+                assert_same(state(inst), ref[k], f"thread form, member {k}")
+        finally:
+            b.close()
+            for inst in insts:
+                close(inst)
+        assert_same(solo(G, SPECS[1], 0, 8), ref[1], "thread form, solo")
+    finally:
+        G.set_interp_mode(0)

@@ -432,3 +432,29 @@ def test_perturbation_uses_splitmix64(oracle):
     assert g.read("um")[second] == um0[second] + 0.01 * (2 * unit[2] - 1)
     assert g.read("vm")[second] == vm0[second] + 0.01 * (2 * unit[3] - 1)
     assert abs(um0[first] - uss) == 0 and abs(vm0[first] - vss) == 0
+
+
+def test_workload_generators():
+    """Seeded workloads: the batching instances and the Poisson job stream
+    have the stated distributions, odd edges, and gels inside their boxes and
+    one element apart from each other."""
+    from glsm_fluid_3d_amd import workloads as W
+
+    inst = W.batch_instances()
+    assert len(inst) == 64 and all(i.edge % 2 == 1 and 33 <= i.edge <= 97 for i in inst)
+    jobs = W.poisson_jobs(duration=600.0)
+    assert abs(len(jobs) / 600.0 - 20.0) < 1.0                      # mean 20 jobs/s
+    gaps = np.diff([0.0] + [j.arrival for j in jobs])
+    assert abs(gaps.std() / gaps.mean() - 1.0) < 0.1                 # exponential gaps
+    assert all(j.edge % 2 == 1 and 41 <= j.edge <= 161 and 1000 <= j.horizon <= 20000 for j in jobs)
+    assert W.poisson_jobs(duration=5.0)[3].arrival == jobs[3].arrival  # seeded
+    for i in inst + jobs[:200]:
+        assert 1 <= len(i.gels) <= 6
+        for n, p, f, ep in i.gels:
+            assert all(n / 2 + 2 - 1e-9 <= x <= i.box - n / 2 - 2 + 1e-9 for x in p)
+        for a in range(len(i.gels)):
+            for b in range(a):
+                (na, pa, _, _), (nb, pb, _, _) = i.gels[a], i.gels[b]
+                assert np.any(np.abs(np.array(pa) - np.array(pb)) >= (na + nb) / 2 + 1 - 1e-9)
+    rates = np.array([(f, ep) for i in inst for _, _, f, ep in i.gels])
+    assert np.all(np.abs(rates[:, 0] / 0.9 - 1) <= 0.2) and np.all(np.abs(rates[:, 1] / 0.3 - 1) <= 0.2)
