@@ -99,6 +99,7 @@ def run_batch(G, storage, steps, warmup, count=64, tune=True):
            "gels": sum(len(g) for g, _, _ in insts), "markers": sum(c.M for _, c, _ in insts),
            "launch": shape, "hip_graph": True,
            "roofline": roofline(cells, 2 * 19 * G.STORAGE_BYTES[storage], lbm_ms)}
+    out["roofline"]["traffic"] = load_traffic(f"batch64_{storage}") if count == 64 else None
     out["roofline"]["kernel"] = "kb_lbm_step (batched persistent pull stream + collide)"
     out["roofline"]["samples"] = samples
     b.close()
@@ -123,7 +124,7 @@ def run_serve(G, seconds, outputs=True, seed=4):
     root = tempfile.mkdtemp(prefix="glsm_serve_") if outputs else None
     srv = S.Server(G, storage="f16", out_root=root, keep_output=False, writers=8)
     try:
-        stats = srv.run(jobs, duration=seconds)
+        stats = srv.run(jobs, duration=seconds, progress_every=30.0)
     finally:
         srv.close()
         if root:

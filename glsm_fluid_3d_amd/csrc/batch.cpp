@@ -247,9 +247,7 @@ void Batch::enqueueStep(bool timed, bool lists) {
     if (timed && k == nsub_ / 2) {
       hipEvent_t e[2];
       for (auto& x : e) GLSM_HIP(hipEventCreate(&x));
-      GLSM_HIP(hipEventRecord(e[0], s));
-      glsm::launch_lbm_step_batch(storage_, t, sel, k == nsub_ - 1, shape, s);
-      GLSM_HIP(hipEventRecord(e[1], s));
+      glsm::launch_lbm_step_batch(storage_, t, sel, k == nsub_ - 1, shape, s, e[0], e[1]);
       timing_events_.push_back(e[0]);
       timing_events_.push_back(e[1]);
     } else {

@@ -252,14 +252,11 @@ void Fluid::stepVelocity(int iter) {
     hipEvent_t ev[2] = {nullptr, nullptr};
     if (timed && kk == Nsub / 2) {
       for (auto& e : ev) GLSM_HIP(hipEventCreate(&e));
-      GLSM_HIP(hipEventRecord(ev[0], fluid_stream));
-    }
-    glsm::launch_lbm_step(storage_, g, p, d_f, d_fnext, plane, d_u, kk == Nsub - 1, sl, m.Fdrag, fluid_stream);
-    if (ev[0]) {
-      GLSM_HIP(hipEventRecord(ev[1], fluid_stream));
       timing_events_.push_back(ev[0]);
       timing_events_.push_back(ev[1]);
     }
+    glsm::launch_lbm_step(storage_, g, p, d_f, d_fnext, plane, d_u, kk == Nsub - 1, sl, m.Fdrag, fluid_stream, ev[0],
+                          ev[1]);
     std::swap(d_f, d_fnext);
   }
 }

@@ -160,9 +160,10 @@ struct LbmParams {
   float inv_scale;
 };
 // one fused pull stream + collide sub-step: fin -> fout (post-collision)
+// ev0/ev1 (optional): start/stop events timestamped by the dispatch itself
 void launch_lbm_step(int storage, const LatticeGeom& g, const LbmParams& p, const void* fin, void* fout,
                      size_t plane, float3* u, bool write_u_all, const SpreadLists& sl, const float3* Fl,
-                     hipStream_t s);
+                     hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 void launch_lbm_init(int storage, const LatticeGeom& g, const LbmParams& p, void* f, size_t plane,
                      float3* u, float* c1, float* c2, hipStream_t s);
 // stored state -> FP32 full populations (pull = pre-collision of the next
@@ -246,7 +247,7 @@ struct LbmLaunch {
 };
 // one sub-step of every member: f[sel] -> f[sel ^ 1]
 void launch_lbm_step_batch(int storage, const BatchTables& t, int sel, bool write_u_all, const LbmLaunch& shape,
-                           hipStream_t s);
+                           hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 // iters[i] += 1 for the n members
 void launch_advance_iters(int* iters, int n, hipStream_t s);
 

@@ -31,6 +31,8 @@
 //   BF16_DEV : d in bfloat16 (round to nearest even).
 #include <type_traits>
 
+#include <hip/hip_ext.h>
+
 #include "glsm_internal.hpp"
 
 namespace glsm {
@@ -499,22 +501,35 @@ __global__ void __launch_bounds__(kThreads) k_copy16(const float4* __restrict__ 
 
 inline int blocks_for(long n) { return (int)((n + kThreads - 1) / kThreads); }
 
+// A timed launch passes start/stop events that the dispatch itself
+// timestamps at kernel start and end (hipExtLaunchKernelGGL), so the sample
+// is the kernel's own duration, without launch latency.
+template <class K, class... A>
+void launch_maybe_timed(K kernel, int grid, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1, A... args) {
+  if (ev0)
+    hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(kThreads), 0, s, ev0, ev1, 0, args...);
+  else
+    kernel<<<grid, kThreads, 0, s>>>(args...);
+}
+
 template <int S>
 void step_impl(const LatticeGeom& g, const LbmParams& p, const void* fin, void* fout, size_t plane, float3* u,
-               bool write_u_all, const SpreadLists& sl, const float3* Fl, hipStream_t s) {
+               bool write_u_all, const SpreadLists& sl, const float3* Fl, hipStream_t s, hipEvent_t ev0,
+               hipEvent_t ev1) {
   using T = typename Store<S>::T;
   const int const_force = (p.Fc.x != 0.f || p.Fc.y != 0.f || p.Fc.z != 0.f) ? 1 : 0;
   // both ping-pong buffers larger than ~3/4 of the 256 MB Infinity Cache:
   // stream the stores past it
   const bool nt = 2.0 * 19.0 * (double)plane * sizeof(T) > 192.0 * (1 << 20);
+  const T* in = static_cast<const T*>(fin);
+  T* out = static_cast<T*>(fout);
+  const int wu = write_u_all ? 1 : 0;
   if (nt)
-    k_lbm_step<S, true><<<blocks_for(g.N), kThreads, 0, s>>>(g, p, static_cast<const T*>(fin), static_cast<T*>(fout),
-                                                             (unsigned)plane, u, write_u_all ? 1 : 0, sl, Fl,
-                                                             const_force);
+    launch_maybe_timed(k_lbm_step<S, true>, blocks_for(g.N), s, ev0, ev1, g, p, in, out, (unsigned)plane, u, wu, sl,
+                       Fl, const_force);
   else
-    k_lbm_step<S, false><<<blocks_for(g.N), kThreads, 0, s>>>(g, p, static_cast<const T*>(fin),
-                                                              static_cast<T*>(fout), (unsigned)plane, u,
-                                                              write_u_all ? 1 : 0, sl, Fl, const_force);
+    launch_maybe_timed(k_lbm_step<S, false>, blocks_for(g.N), s, ev0, ev1, g, p, in, out, (unsigned)plane, u, wu,
+                       sl, Fl, const_force);
 }
 
 }  // namespace
@@ -530,12 +545,12 @@ void launch_copy16(const void* a, void* b, size_t bytes, hipStream_t s) {
 
 void launch_lbm_step(int storage, const LatticeGeom& g, const LbmParams& p, const void* fin, void* fout,
                      size_t plane, float3* u, bool write_u_all, const SpreadLists& sl, const float3* Fl,
-                     hipStream_t s) {
+                     hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
   switch (storage) {
-    case F32_FULL: step_impl<F32_FULL>(g, p, fin, fout, plane, u, write_u_all, sl, Fl, s); break;
-    case F32_DEV: step_impl<F32_DEV>(g, p, fin, fout, plane, u, write_u_all, sl, Fl, s); break;
-    case F16_DEV: step_impl<F16_DEV>(g, p, fin, fout, plane, u, write_u_all, sl, Fl, s); break;
-    case BF16_DEV: step_impl<BF16_DEV>(g, p, fin, fout, plane, u, write_u_all, sl, Fl, s); break;
+    case F32_FULL: step_impl<F32_FULL>(g, p, fin, fout, plane, u, write_u_all, sl, Fl, s, ev0, ev1); break;
+    case F32_DEV: step_impl<F32_DEV>(g, p, fin, fout, plane, u, write_u_all, sl, Fl, s, ev0, ev1); break;
+    case F16_DEV: step_impl<F16_DEV>(g, p, fin, fout, plane, u, write_u_all, sl, Fl, s, ev0, ev1); break;
+    case BF16_DEV: step_impl<BF16_DEV>(g, p, fin, fout, plane, u, write_u_all, sl, Fl, s, ev0, ev1); break;
     default: throw std::invalid_argument("unknown storage mode");
   }
   GLSM_LAUNCH_CHECK();
@@ -551,24 +566,25 @@ void launch_lbm_step(int storage, const LatticeGeom& g, const LbmParams& p, cons
   }
 
 template <int S>
-void step_batch_impl(const BatchTables& t, int sel, bool write_u_all, const LbmLaunch& sh, hipStream_t s) {
+void step_batch_impl(const BatchTables& t, int sel, bool write_u_all, const LbmLaunch& sh, hipStream_t s,
+                     hipEvent_t ev0, hipEvent_t ev1) {
   const int grid = sh.grid > 0 && sh.grid < t.nblk ? sh.grid : t.nblk;
   const int wu = write_u_all ? 1 : 0;
   if (sh.nt && sh.run)
-    kb_lbm_step<S, true, 4><<<grid, kThreads, 0, s>>>(t, sel, wu);
+    launch_maybe_timed(kb_lbm_step<S, true, 4>, grid, s, ev0, ev1, t, sel, wu);
   else if (sh.nt)
-    kb_lbm_step<S, true, 0><<<grid, kThreads, 0, s>>>(t, sel, wu);
+    launch_maybe_timed(kb_lbm_step<S, true, 0>, grid, s, ev0, ev1, t, sel, wu);
   else if (sh.run)
-    kb_lbm_step<S, false, 4><<<grid, kThreads, 0, s>>>(t, sel, wu);
+    launch_maybe_timed(kb_lbm_step<S, false, 4>, grid, s, ev0, ev1, t, sel, wu);
   else
-    kb_lbm_step<S, false, 0><<<grid, kThreads, 0, s>>>(t, sel, wu);
+    launch_maybe_timed(kb_lbm_step<S, false, 0>, grid, s, ev0, ev1, t, sel, wu);
 }
 
 void launch_lbm_step_batch(int storage, const BatchTables& t, int sel, bool write_u_all, const LbmLaunch& shape,
-                           hipStream_t s) {
+                           hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
   if (t.nblk == 0) return;
   if (shape.run != 0 && shape.run != 4) throw std::invalid_argument("XCD run length must be 0 or 4");
-  GLSM_STORAGE_DISPATCH(storage, (step_batch_impl<S>(t, sel, write_u_all, shape, s)));
+  GLSM_STORAGE_DISPATCH(storage, (step_batch_impl<S>(t, sel, write_u_all, shape, s, ev0, ev1)));
   GLSM_LAUNCH_CHECK();
 }
 

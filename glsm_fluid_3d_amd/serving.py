@@ -184,10 +184,15 @@ class Server:
                 shutil.rmtree(rec.out_dir, ignore_errors=True)
 
     # ---- the loop ----------------------------------------------------------
-    def run(self, jobs, duration=None, clock=None, idle_sleep=0.002):
+    def run(self, jobs, duration=None, clock=None, idle_sleep=0.002, progress_every=None, log=None):
         """Serve `jobs` (arrival times relative to the start) until all are
         done or `duration` seconds have passed.  `clock` () -> seconds since
-        start (default: wall time)."""
+        start (default: wall time).  With progress_every, `log` (default:
+        stderr) gets a status line every that many seconds."""
+        import sys
+
+        log = log or (lambda m: print(m, file=sys.stderr, flush=True))
+        next_report = progress_every
         t0 = time.perf_counter()
         now_fn = clock or (lambda: time.perf_counter() - t0)
         pending = deque(sorted(jobs, key=lambda j: j.arrival))
@@ -216,6 +221,11 @@ class Server:
             if clock is not None:
                 clock.advance(n)
             self._retire(now_fn())
+            if progress_every and now >= next_report:
+                next_report += progress_every
+                log(f"[serve {now:7.1f} s] active {len(self.active)} queued {len(self.queue)} "
+                    f"done {len(self.done)} cells {self.active_cells} "
+                    f"MLUPS {self.cell_updates / max(now - self.t_start, 1e-9) / 1e6:.0f}")
         self.elapsed = now_fn() - self.t_start
         if self.output:
             self.output.flush()

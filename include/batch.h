@@ -73,8 +73,9 @@ class Batch {
   // not changed) and keeps the fastest; returns it and its mean launch time.
   Launch autotune(int reps, double* best_ms);
 
-  // Mean duration of the batched stream/collide launch (HIP events around
-  // the middle sub-step of every `every`-th step; 0 = off).
+  // Mean duration of the batched stream/collide kernel (start/stop events
+  // timestamped by the dispatch, middle sub-step of every `every`-th step;
+  // 0 = off).
   void enableLbmTiming(int every);
   double lbmKernelTiming(int* samples);
 

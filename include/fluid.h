@@ -80,10 +80,11 @@ class Fluid {
   int storage() const { return storage_; }
   int storageScaleExp() const { return scale_exp_; }
 
-  // Measurement hook: when enabled, every `every`-th stepVelocity brackets
-  // its middle sub-step's stream/collide launch with a HIP event pair on the
-  // fluid stream (at most `max_samples` pairs).  lbmKernelTiming() waits
-  // for the stream and returns the mean launch duration in ms.
+  // Measurement hook: when enabled, every `every`-th stepVelocity launches
+  // its middle sub-step's stream/collide kernel with start/stop events that
+  // the dispatch timestamps at kernel start and end (at most `max_samples`
+  // pairs).  lbmKernelTiming() waits for the stream and returns the mean
+  // kernel duration in ms.
   void enableLbmTiming(int every, int max_samples = 4096);
   double lbmKernelTiming(int* samples);
 

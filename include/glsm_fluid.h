@@ -169,9 +169,10 @@ enum glsm_storage {
   GLSM_STORAGE_BF16_DEV = 3  /* f - w_q in bfloat16 */
 };
 int glsm_fluid_set_storage(glsm_fluid* f, int storage, int scale_exp);
-/* Extension (measurement): sample the stream/collide launch duration with a
-   HIP event pair on the fluid's stream every `every`-th stepVelocity (0 =
-   off); the result call waits for the stream and returns the mean in ms. */
+/* Extension (measurement): sample the stream/collide kernel duration every
+   `every`-th stepVelocity (0 = off) with events the dispatch timestamps at
+   kernel start and end; the result call waits for the stream and returns
+   the mean in ms. */
 int glsm_fluid_enable_lbm_timing(glsm_fluid* f, int every, int max_samples);
 int glsm_fluid_lbm_timing(glsm_fluid* f, double* mean_ms, int* samples);
 /* lattice (Nx,Ny,Nz) and Nsub (fluid.cu:167-197) */
