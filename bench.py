@@ -38,7 +38,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def config1(mod, storage=None, prec=None, seed=1):
+def cube101_two_gels(mod, storage=None, prec=None, seed=1):
     """101^3 lattice, two 6^3 gels placed as in sim.cpp (x = 1/3, 2/3)."""
     box = (50, 50, 50)
     gels = [mod.Gel((6, 6, 6), (box[0] / 3.0, box[1] / 2.0, box[2] / 2.0), gel_id=1),
@@ -65,7 +65,7 @@ def four_gel_centres(seed=2, box=100.0, n=8, count=4):
     return [tuple(float(v) for v in p) for p in out]
 
 
-def config2b(mod, storage):
+def cube201_four_gels(mod, storage):
     gels = [mod.Gel((8, 8, 8), p, gel_id=i + 1) for i, p in enumerate(four_gel_centres())]
     for i, g in enumerate(gels):
         g.perturb_chemistry(2 + i, 0.01)
@@ -173,7 +173,7 @@ def cpu_baseline(target_s):
 
     threads = min(16, os.cpu_count() or 1)
     O.set_threads(threads)
-    gels, c, f = config1(O, prec=32)
+    gels, c, f = cube101_two_gels(O, prec=32)
     N = f.N
     t0 = time.perf_counter()
     O.system_step(gels, c, f, 0)
@@ -237,14 +237,14 @@ def main():
         return
     if args.workload.startswith("201"):
         st = args.workload.split("-")[1]
-        gels, c, f = config2b(G, st)
+        gels, c, f = cube201_four_gels(G, st)
         dt2, ms2, n2 = run_gpu(G, gels, c, f, args.steps, args.warmup)
         v2 = f.N * f.nsub * args.steps / dt2 / 1e6
         print(json.dumps({"workload": f"201^3, four 8^3 gels, {st}", "value": round(v2, 1), "unit": "MLUPS",
                           "steps": args.steps, "ms_per_step": round(dt2 / args.steps * 1e3, 4),
                           "roofline": roofline(f.N, 2 * 19 * G.STORAGE_BYTES[st], ms2)}), flush=True)
         return
-    gels, c, f = config1(G, storage=args.storage)
+    gels, c, f = cube101_two_gels(G, storage=args.storage)
     N = f.N
     log(f"workload 101^3: N={N} nsub={f.nsub} markers={c.M} storage={args.storage}")
     dt, lbm_ms, samples = run_gpu(G, gels, c, f, args.steps, args.warmup)
@@ -260,7 +260,7 @@ def main():
                    "lattice": [101, 101, 101], "gels": 2, "markers": c.M, "nsub": f.nsub,
                    "storage": args.storage + (" (f - w_q deviations)" if args.storage != "f32full" else ""),
                    "parallelism": "single GPU"},
-        "roofline": roofline(N, bpc, lbm_ms, load_traffic(f"config1_{args.storage}")),
+        "roofline": roofline(N, bpc, lbm_ms, load_traffic(f"cube101_two_gels_{args.storage}")),
     }
     line["roofline"]["samples"] = samples
     log(f"101^3: {mlups:.0f} MLUPS, {dt / args.steps * 1e3:.3f} ms/step, lbm kernel {lbm_ms * 1e3:.1f} us")
@@ -273,13 +273,13 @@ def main():
     if not args.no_extra:
         extra = {}
         for st in ("f32", "f16"):
-            gels, c, f = config2b(G, st)
+            gels, c, f = cube201_four_gels(G, st)
             dt2, ms2, n2 = run_gpu(G, gels, c, f, args.extra_steps, 20)
             v2 = f.N * f.nsub * args.extra_steps / dt2 / 1e6
             extra[f"201cube_4gels_{st}"] = {
                 "value": round(v2, 1), "unit": "MLUPS", "steps": args.extra_steps,
                 "ms_per_step": round(dt2 / args.extra_steps * 1e3, 4),
-                "roofline": roofline(f.N, 2 * 19 * G.STORAGE_BYTES[st], ms2, load_traffic(f"config2b_{st}"))}
+                "roofline": roofline(f.N, 2 * 19 * G.STORAGE_BYTES[st], ms2, load_traffic(f"cube201_four_gels_{st}"))}
             log(f"201^3 {st}: {v2:.0f} MLUPS, lbm kernel {ms2 * 1e3:.1f} us")
             for x in gels + [c, f]:
                 x.close()

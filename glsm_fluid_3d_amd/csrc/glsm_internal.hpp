@@ -22,6 +22,47 @@ void set_device(int device);
   } while (0)
 #define GLSM_LAUNCH_CHECK() GLSM_HIP(hipGetLastError())
 
+// Device global memory (address space 1).  A pointer read from a device-side
+// table (the batch descriptors) is a generic pointer to the compiler, which
+// then emits flat memory instructions: each is counted against both the
+// vector-memory and the scalar/LDS wait counters, so a wait for a scalar
+// descriptor load also waits for every population load in flight.  Every
+// buffer of this library is global memory; accessing it through GLSM_GLOBAL
+// pointers gives global_* instructions (SGPR base + 32-bit lane offset).
+// (The host pass parses device code too; it has no such address space.)
+#ifdef __HIP_DEVICE_COMPILE__
+#define GLSM_GLOBAL __attribute__((address_space(1)))
+#else
+#define GLSM_GLOBAL
+#endif
+template <class T>
+__device__ __forceinline__ GLSM_GLOBAL T* gmem(T* p) {
+  return (GLSM_GLOBAL T*)p;
+}
+
+// Work-groups are dealt to the 8 XCDs round-robin (id mod 8), and each XCD
+// has its own L2.  Consecutive 256-cell blocks share the edge cache line of
+// every shifted plane read, so the logical block of a work-group is chosen
+// to put neighbours on the same XCD, where they meet in one L2 instead of
+// fetching the shared line twice (measured: 1.10x -> 1.01x of the
+// algorithmic bytes at 201^3).
+//   RUN = 0: one contiguous run per XCD -- best while the state lives in the
+//            Infinity Cache;
+//   RUN > 0: runs of RUN blocks, dealt round-robin -- keeps the DRAM access
+//            front compact for HBM-resident lattices (RUN = 4 measured best).
+template <int RUN>
+__device__ __forceinline__ unsigned xcd_block(unsigned id, unsigned nb) {
+  constexpr unsigned kXcd = 8;
+  if (RUN == 0) {
+    const unsigned q = nb / kXcd, r = nb % kXcd, x = id % kXcd, k = id / kXcd;
+    return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+  }
+  const unsigned grp = kXcd * RUN, full = nb / grp * grp;
+  if (id >= full) return id;
+  const unsigned base = id / grp * grp, w = id - base;
+  return base + (w % kXcd) * RUN + w / kXcd;
+}
+
 template <class T>
 T* dalloc(size_t n) {  // zero-filled device allocation
   void* p = nullptr;
@@ -235,6 +276,7 @@ void launch_ibm_interp_batch(const BatchTables& t, hipStream_t s);  // beta_eff 
 void launch_ibm_sample_batch(const BatchTables& t, int cpar, hipStream_t s);
 void launch_spread_rebuild_batch(const BatchTables& t, hipStream_t s);
 void launch_scalar_batch(const BatchTables& t, int cpar, hipStream_t s);
+
 
 // Launch shape of the batched stream/collide kernel (the autotuner's knobs):
 // non-temporal stores, XCD run length (0 = one contiguous run per XCD) and

@@ -19,6 +19,9 @@
 //     scan, fill, sort).  The fused LBM kernel and the scalar kernel sum a
 //     cell's contributions in marker order -- no atomics, the same bits every
 //     run.
+#include <algorithm>
+#include <cstddef>
+
 #include "glsm_internal.hpp"
 
 namespace glsm {
@@ -223,6 +226,9 @@ __device__ __forceinline__ void interp_wave(const MarkerArrays& m, const SpreadL
 __device__ __forceinline__ float stencil_sum(const MarkerArrays& m, const SpreadLists& sl,
                                              const float* __restrict__ field, int stride, int comp, int l) {
   constexpr int kChunk = 16;  // table entries, then field values, loaded ahead per chunk
+  const GLSM_GLOBAL int* ctab = gmem(sl.ctab);
+  const GLSM_GLOBAL float* rtab = gmem(sl.rtab);
+  const GLSM_GLOBAL float* gf = gmem(field);
   float acc = 0.f;
   for (int k0 = 0; k0 < 64; k0 += kChunk) {
     int cell[kChunk];
@@ -230,11 +236,11 @@ __device__ __forceinline__ float stencil_sum(const MarkerArrays& m, const Spread
 #pragma unroll
     for (int j = 0; j < kChunk; ++j) {
       const size_t t = tab(m, l, k0 + j);
-      cell[j] = max(sl.ctab[t], 0);
-      w[j] = sl.rtab[t];
+      cell[j] = max(ctab[t], 0);
+      w[j] = rtab[t];
     }
 #pragma unroll
-    for (int j = 0; j < kChunk; ++j) f[j] = field[(size_t)stride * cell[j] + comp];
+    for (int j = 0; j < kChunk; ++j) f[j] = gf[(size_t)stride * cell[j] + comp];
 #pragma unroll
     for (int j = 0; j < kChunk; ++j) acc += f[j] * w[j];
   }
@@ -408,15 +414,18 @@ __device__ inline void decode_cell(const LatticeGeom& g, int i, int& x, int& y, 
 // first-order upwind advection + 7-point diffusion + spread IBM source,
 // forward Euler (k_zero_scalar + k_ibm_spread_concentration +
 // k_convection_diffusion, fluid_kernels.cu:119-123, 414-459, 177-242)
+static_assert(sizeof(SpreadEntry) == 8 && offsetof(SpreadEntry, w) == 4, "SpreadEntry is read as (int, float bits)");
 __device__ __forceinline__ void scalar_at(const LatticeGeom& g, const ScalarParams& p, const float3* __restrict__ u,
                                           const float* __restrict__ c1, float* __restrict__ c2, const SpreadLists& sl,
                                           const float* __restrict__ Sl, int b) {
   const int i = b * kFlagCells + threadIdx.x;
-  const int flagged = sl.blkflag[b];
+  const int flagged = gmem(sl.blkflag)[b];
   if (i >= g.N) return;
   int x, y, z;
   decode_cell(g, i, x, y, z);
   const int Nx = g.Nx, Ny = g.Ny, Nz = g.Nz;
+  const GLSM_GLOBAL float* gu = gmem(reinterpret_cast<const float*>(u));
+  const GLSM_GLOBAL float* gc = gmem(c1);
   const int xp = (x + 1 < Nx) ? x + 1 : (g.perX ? 0 : x);
   const int xm = (x - 1 >= 0) ? x - 1 : (g.perX ? Nx - 1 : x);
   const int yp = (y + 1 < Ny) ? y + 1 : (g.perY ? 0 : y);
@@ -429,25 +438,39 @@ __device__ __forceinline__ void scalar_at(const LatticeGeom& g, const ScalarPara
   const int izp = x + y * Nx + zp * sxy, izm = x + y * Nx + zm * sxy;
   float src = 0.f;
   if (flagged) {
-    const int2 r = sl.rng[i];
-    for (int k = r.x; k < r.x + r.y; ++k) {
-      const SpreadEntry e = sl.ent[k];
-      src += Sl[e.marker] * e.w;
+    const GLSM_GLOBAL int* rp = gmem(reinterpret_cast<const int*>(sl.rng)) + 2 * i;
+    const GLSM_GLOBAL int* ent = gmem(reinterpret_cast<const int*>(sl.ent));  // (marker, w bits) pairs
+    const GLSM_GLOBAL float* gS = gmem(Sl);
+    const int e0 = rp[0], e1 = e0 + rp[1];
+    for (int k = e0; k < e1; k += 4) {  // four entries' loads ahead of the ordered adds
+      int mk[4];
+      float w[4], sv[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int kk = min(k + j, e1 - 1);
+        mk[j] = ent[2 * kk];
+        w[j] = __int_as_float(ent[2 * kk + 1]);
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) sv[j] = gS[mk[j]];
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (k + j < e1) src += sv[j] * w[j];
     }
   }
   const float ihx = 1.0f / p.dx, ihy = 1.0f / p.dy, ihz = 1.0f / p.dz;
   const float ihx2 = ihx * ihx, ihy2 = ihy * ihy, ihz2 = ihz * ihz;
-  const float c = c1[i];
-  const float cxp = c1[ixp], cxm = c1[ixm], cyp = c1[iyp], cym = c1[iym], czp = c1[izp], czm = c1[izm];
-  const float3 u0 = u[i];
-  const float uL = 0.5f * (u[ixm].x + u0.x), uR = 0.5f * (u0.x + u[ixp].x);
-  const float vL = 0.5f * (u[iym].y + u0.y), vR = 0.5f * (u0.y + u[iyp].y);
-  const float wL = 0.5f * (u[izm].z + u0.z), wR = 0.5f * (u0.z + u[izp].z);
+  const float c = gc[i];
+  const float cxp = gc[ixp], cxm = gc[ixm], cyp = gc[iyp], cym = gc[iym], czp = gc[izp], czm = gc[izm];
+  const float u0x = gu[3 * i], u0y = gu[3 * i + 1], u0z = gu[3 * i + 2];
+  const float uL = 0.5f * (gu[3 * ixm] + u0x), uR = 0.5f * (u0x + gu[3 * ixp]);
+  const float vL = 0.5f * (gu[3 * iym + 1] + u0y), vR = 0.5f * (u0y + gu[3 * iyp + 1]);
+  const float wL = 0.5f * (gu[3 * izm + 2] + u0z), wR = 0.5f * (u0z + gu[3 * izp + 2]);
   const float FLx = uL * ((uL >= 0.0f) ? cxm : c), FLy = vL * ((vL >= 0.0f) ? cym : c), FLz = wL * ((wL >= 0.0f) ? czm : c);
   const float FRx = uR * ((uR >= 0.0f) ? c : cxp), FRy = vR * ((vR >= 0.0f) ? c : cyp), FRz = wR * ((wR >= 0.0f) ? c : czp);
   const float conv = -((FRx - FLx) * ihx + (FRy - FLy) * ihy + (FRz - FLz) * ihz);
   const float diff = p.D * ((cxm - 2.0f * c + cxp) * ihx2 + (cym - 2.0f * c + cyp) * ihy2 + (czm - 2.0f * c + czp) * ihz2);
-  c2[i] = c + p.dt * (conv + diff + src);
+  gmem(c2)[i] = c + p.dt * (conv + diff + src);
 }
 
 // ---------------------------------------------------------------- kernels
@@ -506,7 +529,7 @@ __global__ void __launch_bounds__(kThreads) k_spread_sort(LatticeGeom g, SpreadL
 __global__ void __launch_bounds__(kThreads) k_scalar(LatticeGeom g, ScalarParams p, const float3* __restrict__ u,
                                                      const float* __restrict__ c1, float* __restrict__ c2,
                                                      SpreadLists sl, const float* __restrict__ Sl) {
-  scalar_at(g, p, u, c1, c2, sl, Sl, blockIdx.x);
+  scalar_at(g, p, u, c1, c2, sl, Sl, xcd_block<0>(blockIdx.x, gridDim.x));
 }
 
 // Batched kernels: markers and 256-cell blocks of all members are numbered
@@ -621,9 +644,11 @@ __global__ void __launch_bounds__(kThreads) kb_spread_sort(BatchTables t) {
   const BatchInst& I = block_member(t, blockIdx.x, b);
   sort_at(I.g, I.sl, b);
 }
+// contiguous runs of blocks per XCD: the y/z neighbour rows a block reads
+// were mostly fetched by the same XCD's L2
 __global__ void __launch_bounds__(kThreads) kb_scalar(BatchTables t, int cpar) {
   int b;
-  const BatchInst& I = block_member(t, blockIdx.x, b);
+  const BatchInst& I = block_member(t, xcd_block<0>(blockIdx.x, gridDim.x), b);
   scalar_at(I.g, I.sp, I.u, I.c[cpar], I.c[cpar ^ 1], I.sl, I.m.Sl, b);
 }
 

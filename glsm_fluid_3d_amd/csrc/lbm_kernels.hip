@@ -29,6 +29,7 @@
 //              keep ~1e-7 relative precision on it;
 //   F16_DEV  : d * 2^s in binary16 (s = scale exponent, default 20);
 //   BF16_DEV : d in bfloat16 (round to nearest even).
+#include <cstddef>
 #include <type_traits>
 
 #include <hip/hip_ext.h>
@@ -49,13 +50,20 @@ constexpr int OPP[19] = {0, 2, 1, 4, 3, 6, 5, 9, 10, 7, 8, 13, 14, 11, 12, 17, 1
 // gfx950.  Stores are non-temporal only when the two distribution buffers
 // exceed the 256 MB Infinity Cache (NT): below that, plain stores leave the
 // next sub-step's input in the cache.
-template <class T, bool NT>
-__device__ __forceinline__ void put(T* p, T v) {
+template <class T, bool NT, class P>
+__device__ __forceinline__ void put(P p, T v) {
   if (NT)
     __builtin_nontemporal_store(v, p);
   else
     *p = v;
 }
+// plain 3-float record for u and the precomputed cell forces: trivially
+// copyable, so it is read and written through GLSM_GLOBAL pointers as one
+// 12-byte access
+struct Vec3 {
+  float x, y, z;
+};
+
 template <int S>
 struct Store;
 // Values move in stored units: f (full storage), d = f - w (deviations),
@@ -65,9 +73,12 @@ template <>
 struct Store<F32_FULL> {
   using T = float;
   static constexpr bool dev = false, scaled = false;
-  __device__ static float ld(const T* p) { return *p; }
-  template <bool NT>
-  __device__ static float st(T* p, float v) {
+  template <class P>
+  __device__ static float ld(P p) {
+    return *p;
+  }
+  template <bool NT, class P>
+  __device__ static float st(P p, float v) {
     put<T, NT>(p, v);
     return v;
   }
@@ -76,9 +87,12 @@ template <>
 struct Store<F32_DEV> {
   using T = float;
   static constexpr bool dev = true, scaled = false;
-  __device__ static float ld(const T* p) { return *p; }
-  template <bool NT>
-  __device__ static float st(T* p, float v) {
+  template <class P>
+  __device__ static float ld(P p) {
+    return *p;
+  }
+  template <bool NT, class P>
+  __device__ static float st(P p, float v) {
     put<T, NT>(p, v);
     return v;
   }
@@ -87,9 +101,12 @@ template <>
 struct Store<F16_DEV> {
   using T = _Float16;
   static constexpr bool dev = true, scaled = true;
-  __device__ static float ld(const T* p) { return (float)*p; }
-  template <bool NT>
-  __device__ static float st(T* p, float v) {
+  template <class P>
+  __device__ static float ld(P p) {
+    return (float)*p;
+  }
+  template <bool NT, class P>
+  __device__ static float st(P p, float v) {
     const _Float16 h = (_Float16)v;
     put<T, NT>(p, h);
     return (float)h;
@@ -99,9 +116,12 @@ template <>
 struct Store<BF16_DEV> {
   using T = uint16_t;
   static constexpr bool dev = true, scaled = false;
-  __device__ static float ld(const T* p) { return __uint_as_float((uint32_t)*p << 16); }
-  template <bool NT>
-  __device__ static float st(T* p, float v) {
+  template <class P>
+  __device__ static float ld(P p) {
+    return __uint_as_float((uint32_t)*p << 16);
+  }
+  template <bool NT, class P>
+  __device__ static float st(P p, float v) {
     const uint32_t b = __float_as_uint(v);
     const uint32_t r = (b + 0x7FFFu + ((b >> 16) & 1u)) >> 16;  // round to nearest even (finite v)
     put<T, NT>(p, (uint16_t)r);
@@ -175,29 +195,6 @@ struct PullAddr {
   }
 };
 
-// Work-groups are dealt to the 8 XCDs round-robin (id mod 8), and each XCD
-// has its own L2.  Consecutive 256-cell blocks share the edge cache line of
-// every shifted plane read, so the logical block of a work-group is chosen
-// to put neighbours on the same XCD, where they meet in one L2 instead of
-// fetching the shared line twice (measured: 1.10x -> 1.01x of the
-// algorithmic bytes at 201^3).
-//   RUN = 0: one contiguous run per XCD -- best while the state lives in the
-//            Infinity Cache;
-//   RUN > 0: runs of RUN blocks, dealt round-robin -- keeps the DRAM access
-//            front compact for HBM-resident lattices (RUN = 4 measured best).
-template <int RUN>
-__device__ __forceinline__ unsigned xcd_block(unsigned id, unsigned nb) {
-  constexpr unsigned kXcd = 8;
-  if (RUN == 0) {
-    const unsigned q = nb / kXcd, r = nb % kXcd, x = id % kXcd, k = id / kXcd;
-    return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
-  }
-  const unsigned grp = kXcd * RUN, full = nb / grp * grp;
-  if (id >= full) return id;
-  const unsigned base = id / grp * grp, w = id - base;
-  return base + (w % kXcd) * RUN + w / kXcd;
-}
-
 template <int q>
 struct Unroll {
   template <class F>
@@ -229,8 +226,8 @@ __device__ __forceinline__ float wq(const LbmParams& p, int q) { return q == 0 ?
 // accumulating 19 roundings per sub-step.
 template <int S, bool FORCED, bool NT>
 __device__ __forceinline__ void collide_store(const float (&v)[19], float Fx, float Fy, float Fz, const LbmParams& p,
-                                              typename Store<S>::T* __restrict__ fout, unsigned plane, unsigned i,
-                                              float3* u, bool store_u) {
+                                              GLSM_GLOBAL typename Store<S>::T* __restrict__ fout, unsigned plane,
+                                              unsigned i, GLSM_GLOBAL Vec3* __restrict__ u, bool store_u) {
   constexpr bool DEV = Store<S>::dev;
   constexpr bool MOM64 = S == F16_DEV || S == BF16_DEV;
   constexpr bool SCALED = Store<S>::scaled;
@@ -273,7 +270,7 @@ __device__ __forceinline__ void collide_store(const float (&v)[19], float Fx, fl
   }
   const float inv_rho = isc / rho;
   const float ux = mx * inv_rho, uy = my * inv_rho, uz = mz * inv_rho;
-  if (store_u) u[i] = make_float3(ux, uy, uz);
+  if (store_u) u[i] = Vec3{ux, uy, uz};
   // The multiply-adds below are written as fmaf: each rounds once, the same
   // way in every kernel this body is inlined into.
   const float omega = p.omega;
@@ -304,6 +301,9 @@ __device__ __forceinline__ void collide_store(const float (&v)[19], float Fx, fl
   if (MOM64) Store<S>::template st<NT>(fout + i, (float)(sd - (double)kept));
 }
 
+// Every access goes through GLSM_GLOBAL pointers: the batched kernels take
+// their pointers from a device-side table, which would otherwise compile to
+// flat memory instructions.
 template <int S, bool NT>
 __device__ __forceinline__ void lbm_block(const LatticeGeom& g, const LbmParams& p,
                                           const typename Store<S>::T* __restrict__ fin,
@@ -311,33 +311,59 @@ __device__ __forceinline__ void lbm_block(const LatticeGeom& g, const LbmParams&
                                           float3* __restrict__ u, int write_u_all, const SpreadLists& sl,
                                           const float3* __restrict__ Fl, int const_force, unsigned b) {
   const unsigned i = b * kThreads + threadIdx.x;
-  const int flagged = sl.blkflag[b];
+  const int flagged = gmem(sl.blkflag)[b];
   if (i >= (unsigned)g.N) return;
   int x, y, z;
   decode_cell(g, (int)i, x, y, z);
   const PullAddr a(g, x, y, z);
+  const GLSM_GLOBAL typename Store<S>::T* gin = gmem(fin);
   float v[19];
   Unroll<19>::run([&](auto qc) {
     constexpr int q = decltype(qc)::value;
-    v[q] = Store<S>::ld(fin + a.src<q>(i, plane));
+    v[q] = Store<S>::ld(gin + a.src<q>(i, plane));
   });
   const bool store_u = write_u_all || flagged;
+  GLSM_GLOBAL Vec3* gu = gmem(reinterpret_cast<Vec3*>(u));
   if (flagged || const_force) {
     float Fx = 0.f, Fy = 0.f, Fz = 0.f;
     if (flagged) {
+      // sum of Fl[marker] * w over the cell's list, in list (marker) order;
+      // four entries, then their four marker forces, are loaded ahead of
+      // the ordered adds (an entry past the end re-reads the last one and
+      // is not added).  Products and sums are rounded one by one.
 #pragma clang fp contract(off)
-      const int2 r = sl.rng[i];
-      for (int k = r.x; k < r.x + r.y; ++k) {
-        const SpreadEntry e = sl.ent[k];
-        const float3 fl = Fl[e.marker];
-        Fx += fl.x * e.w;
-        Fy += fl.y * e.w;
-        Fz += fl.z * e.w;
+      const GLSM_GLOBAL int* rp = gmem(reinterpret_cast<const int*>(sl.rng)) + 2 * i;
+      const int e0 = rp[0], e1 = e0 + rp[1];
+      static_assert(sizeof(SpreadEntry) == 8 && offsetof(SpreadEntry, w) == 4, "entries are (int, float bits)");
+      const GLSM_GLOBAL int* ent = gmem(reinterpret_cast<const int*>(sl.ent));
+      const GLSM_GLOBAL float* gF = gmem(reinterpret_cast<const float*>(Fl));
+      for (int k = e0; k < e1; k += 4) {
+        int mk[4];
+        float w[4], fx[4], fy[4], fz[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int kk = min(k + j, e1 - 1);
+          mk[j] = ent[2 * kk];
+          w[j] = __int_as_float(ent[2 * kk + 1]);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          fx[j] = gF[3 * mk[j]];
+          fy[j] = gF[3 * mk[j] + 1];
+          fz[j] = gF[3 * mk[j] + 2];
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          if (k + j < e1) {
+            Fx += fx[j] * w[j];
+            Fy += fy[j] * w[j];
+            Fz += fz[j] * w[j];
+          }
       }
     }
-    collide_store<S, true, NT>(v, p.Fc.x + Fx, p.Fc.y + Fy, p.Fc.z + Fz, p, fout, plane, i, u, store_u);
+    collide_store<S, true, NT>(v, p.Fc.x + Fx, p.Fc.y + Fy, p.Fc.z + Fz, p, gmem(fout), plane, i, gu, store_u);
   } else {
-    collide_store<S, false, NT>(v, 0.f, 0.f, 0.f, p, fout, plane, i, u, store_u);
+    collide_store<S, false, NT>(v, 0.f, 0.f, 0.f, p, gmem(fout), plane, i, gu, store_u);
   }
 }
 

@@ -1,4 +1,4 @@
-"""Writes tests/golden/config0.npz: the FP64 oracle's state for the small
+"""Writes tests/golden/cube41_one_gel.npz: the FP64 oracle's state for the small
 plain-method workload -- a 41^3 D3Q19 lattice (physical box 20^3, h = 0.5),
 periodic on all axes, one 6x6x6-element gel centred at (10,10,10), chemistry
 at the steady state plus a splitmix64(seed 0) perturbation of amplitude 0.01,
@@ -9,7 +9,7 @@ Stored: the gel's node positions and element chemistry (float64) at both
 checkpoints, and the velocity and concentration fields at step 1000 (rounded
 to float32, far below the tolerances they are compared with).  Run from the
 repository root:
-    python tests/golden/make_config0.py
+    python tests/golden/make_cube41_one_gel.py
 """
 import os
 import sys
@@ -21,7 +21,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__)
 sys.path.insert(0, ROOT)
 import oracle as O  # noqa: E402
 
-OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "config0.npz")
+OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cube41_one_gel.npz")
 CHECKPOINTS = (100, 1000)
 
 

@@ -14,7 +14,7 @@ Tolerances:
   * the small plain-method workload (41^3, one gel, periodic) against FP64:
     velocity rel-L2 <= 1e-5, concentration rel-L2 <= 1e-6, gel node
     positions within 5e-7 gel units (= 1e-6 lattice units) after 200 live
-    steps and after 1000 steps against tests/golden/config0.npz.
+    steps and after 1000 steps against tests/golden/cube41_one_gel.npz.
 """
 import os
 import sys
@@ -234,7 +234,7 @@ def test_concentration_step_bit_exact(glsm, oracle):
 
 
 # ------------------------------------------------------------------ whole steps
-def config0(mod, prec=None, storage=None):
+def cube41_one_gel(mod, prec=None, storage=None):
     g = mod.Gel((6, 6, 6), (10, 10, 10))
     c = mod.Coupler([g]) if prec is None else mod.Coupler([g], prec)
     f = mod.Fluid((20, 20, 20), c)
@@ -245,10 +245,10 @@ def config0(mod, prec=None, storage=None):
     return g, c, f
 
 
-def test_config0_200_steps_vs_fp64(glsm, oracle):
-    say("config0 200 steps")
-    og, oc, of = config0(oracle, 64)
-    sims = {st: config0(glsm, storage=st) for st in ("f32", "f16", "bf16")}
+def test_cube41_200_steps_vs_fp64(glsm, oracle):
+    say("41^3 one gel, 200 steps")
+    og, oc, of = cube41_one_gel(oracle, 64)
+    sims = {st: cube41_one_gel(glsm, storage=st) for st in ("f32", "f16", "bf16")}
     for it in range(200):
         oracle.system_step([og], oc, of, it)
         for g, c, f in sims.values():
@@ -268,10 +268,10 @@ def test_config0_200_steps_vs_fp64(glsm, oracle):
     assert rel_l2(g.read("un_norm"), og.read("un_norm")) <= 1e-6
 
 
-def test_config0_1000_steps_vs_golden(glsm):
-    say("config0 1000 steps vs golden")
-    gold = np.load(os.path.join(HERE, "golden", "config0.npz"))
-    g, c, f = config0(glsm, storage="f32")
+def test_cube41_1000_steps_vs_golden(glsm):
+    say("41^3 one gel, 1000 steps vs golden")
+    gold = np.load(os.path.join(HERE, "golden", "cube41_one_gel.npz"))
+    g, c, f = cube41_one_gel(glsm, storage="f32")
     for it in range(1000):
         glsm.system_step([g], c, f, it)
         if it + 1 == 100:
@@ -303,7 +303,7 @@ def test_runs_are_bitwise_deterministic(glsm):
     say("determinism")
     outs = []
     for _ in range(2):
-        g, c, f = config0(glsm)
+        g, c, f = cube41_one_gel(glsm)
         for it in range(30):
             glsm.system_step([g], c, f, it)
         outs.append((f.read("u"), f.read("c"), f.read("fpost"), g.read("rn"), g.read("um")))
