@@ -89,6 +89,10 @@ int Batch::add(std::vector<Gel*>& gels, Coupler* coupler, Fluid* fluid, int iter
     if (m.fluid == fluid || m.coupler == coupler) throw std::invalid_argument("Batch::add: instance already a member");
   if (nsub_ == 0) nsub_ = fluid->Nsub;
   if (fluid->Nsub != nsub_) throw std::invalid_argument("Batch::add: members must share the sub-step count");
+  // the batched stream/collide kernel addresses a member's distributions by
+  // 32-bit byte offsets
+  if (19ull * fluid->plane * glsm::storage_bytes(storage_) >= (1ull << 32))
+    throw std::invalid_argument("Batch::add: lattice too large for a batch member (distributions over 4 GiB)");
   if (fluid->storage() != storage_ || (storage_ == Fluid::F16_DEV && fluid->storageScaleExp() != scale_exp_))
     fluid->setStorage(storage_, scale_exp_);
   GLSM_HIP(hipStreamSynchronize(glsm::stream()));

@@ -211,6 +211,16 @@ struct Unroll<0> {
 
 __device__ __forceinline__ float wq(const LbmParams& p, int q) { return q == 0 ? p.w0 : (q < 7 ? p.w1 : p.w2); }
 
+// Element `idx` of a population buffer.  B32 (buffer below 4 GiB): the base
+// plus a 32-bit byte offset, which the load or store takes as an SGPR base +
+// VGPR offset instead of a 64-bit lane address; otherwise 64-bit indexing.
+template <bool B32, class T>
+__device__ __forceinline__ T* elem_at(T* f, unsigned idx) {
+  using C = typename std::conditional<std::is_const<T>::value, const GLSM_GLOBAL char, GLSM_GLOBAL char>::type;
+  if (B32) return reinterpret_cast<T*>(reinterpret_cast<C*>(f) + idx * (unsigned)sizeof(T));
+  return f + idx;
+}
+
 // BGK + Guo on one cell.  v[] holds the stored units (f, d = f - w, or d
 // scaled by S = 2^scale_exp); the same units are written back.  The scale
 // enters per cell, not per population: rho = 1 + s/S, u = m / (rho S), the
@@ -224,7 +234,7 @@ __device__ __forceinline__ float wq(const LbmParams& p, int q) { return q == 0 ?
 // remainder of the cell's mass after the 18 moving ones were rounded: a cell
 // keeps its stored mass up to the one rounding of that remainder instead of
 // accumulating 19 roundings per sub-step.
-template <int S, bool FORCED, bool NT>
+template <int S, bool FORCED, bool NT, bool B32>
 __device__ __forceinline__ void collide_store(const float (&v)[19], float Fx, float Fy, float Fz, const LbmParams& p,
                                               GLSM_GLOBAL typename Store<S>::T* __restrict__ fout, unsigned plane,
                                               unsigned i, GLSM_GLOBAL Vec3* __restrict__ u, bool store_u) {
@@ -295,16 +305,16 @@ __device__ __forceinline__ void collide_store(const float (&v)[19], float Fx, fl
       const float ciF = cx * Fx + cy * Fy + cz * Fz;
       out = fmaf(SCALED ? w * kF * sc : w * kF, fmaf(9.0f * cu, ciF, 3.0f * (ciF - uF)), out);
     }
-    const float st = Store<S>::template st<NT>(fout + (unsigned)q * plane + i, out);
+    const float st = Store<S>::template st<NT>(elem_at<B32>(fout, (unsigned)q * plane + i), out);
     if (MOM64) kept += st;
   });
-  if (MOM64) Store<S>::template st<NT>(fout + i, (float)(sd - (double)kept));
+  if (MOM64) Store<S>::template st<NT>(elem_at<B32>(fout, i), (float)(sd - (double)kept));
 }
 
 // Every access goes through GLSM_GLOBAL pointers: the batched kernels take
 // their pointers from a device-side table, which would otherwise compile to
 // flat memory instructions.
-template <int S, bool NT>
+template <int S, bool NT, bool B32>
 __device__ __forceinline__ void lbm_block(const LatticeGeom& g, const LbmParams& p,
                                           const typename Store<S>::T* __restrict__ fin,
                                           typename Store<S>::T* __restrict__ fout, unsigned plane,
@@ -320,7 +330,7 @@ __device__ __forceinline__ void lbm_block(const LatticeGeom& g, const LbmParams&
   float v[19];
   Unroll<19>::run([&](auto qc) {
     constexpr int q = decltype(qc)::value;
-    v[q] = Store<S>::ld(gin + a.src<q>(i, plane));
+    v[q] = Store<S>::ld(elem_at<B32>(gin, a.src<q>(i, plane)));
   });
   const bool store_u = write_u_all || flagged;
   GLSM_GLOBAL Vec3* gu = gmem(reinterpret_cast<Vec3*>(u));
@@ -361,21 +371,21 @@ __device__ __forceinline__ void lbm_block(const LatticeGeom& g, const LbmParams&
           }
       }
     }
-    collide_store<S, true, NT>(v, p.Fc.x + Fx, p.Fc.y + Fy, p.Fc.z + Fz, p, gmem(fout), plane, i, gu, store_u);
+    collide_store<S, true, NT, B32>(v, p.Fc.x + Fx, p.Fc.y + Fy, p.Fc.z + Fz, p, gmem(fout), plane, i, gu, store_u);
   } else {
-    collide_store<S, false, NT>(v, 0.f, 0.f, 0.f, p, gmem(fout), plane, i, gu, store_u);
+    collide_store<S, false, NT, B32>(v, 0.f, 0.f, 0.f, p, gmem(fout), plane, i, gu, store_u);
   }
 }
 
 // 8 waves per SIMD (<= 64 VGPRs): keeps enough loads in flight per CU
-template <int S, bool NT>
+template <int S, bool NT, bool B32>
 __global__ void __launch_bounds__(kThreads, 8) k_lbm_step(LatticeGeom g, LbmParams p,
                                                        const typename Store<S>::T* __restrict__ fin,
                                                        typename Store<S>::T* __restrict__ fout, unsigned plane,
                                                        float3* __restrict__ u, int write_u_all, SpreadLists sl,
                                                        const float3* __restrict__ Fl, int const_force) {
   const unsigned b = xcd_block<NT ? 4 : 0>(blockIdx.x, gridDim.x);
-  lbm_block<S, NT>(g, p, fin, fout, plane, u, write_u_all, sl, Fl, const_force, b);
+  lbm_block<S, NT, B32>(g, p, fin, fout, plane, u, write_u_all, sl, Fl, const_force, b);
 }
 
 // All members of a batch: the blocks of every lattice form one flattened
@@ -388,8 +398,9 @@ __global__ void __launch_bounds__(kThreads, 8) kb_lbm_step(BatchTables t, int se
   for (unsigned w = blockIdx.x; w < (unsigned)t.nblk; w += gridDim.x) {
     const unsigned gb = xcd_block<RUN>(w, (unsigned)t.nblk);
     const BatchInst& I = t.inst[t.blk_inst[gb]];
-    lbm_block<S, NT>(I.g, I.p, static_cast<const T*>(I.f[sel]), static_cast<T*>(I.f[sel ^ 1]), (unsigned)I.plane,
-                     I.u, write_u_all, I.sl, I.m.Fdrag, I.const_force, gb - (unsigned)I.blk0);
+    lbm_block<S, NT, true>(I.g, I.p, static_cast<const T*>(I.f[sel]), static_cast<T*>(I.f[sel ^ 1]),
+                           (unsigned)I.plane, I.u, write_u_all, I.sl, I.m.Fdrag, I.const_force,
+                           gb - (unsigned)I.blk0);
   }
 }
 
@@ -538,6 +549,9 @@ void launch_maybe_timed(K kernel, int grid, hipStream_t s, hipEvent_t ev0, hipEv
     kernel<<<grid, kThreads, 0, s>>>(args...);
 }
 
+// a distribution buffer whose every byte offset fits in 32 bits
+bool fits_b32(size_t plane, size_t elem) { return 19ull * plane * elem < (1ull << 32); }
+
 template <int S>
 void step_impl(const LatticeGeom& g, const LbmParams& p, const void* fin, void* fout, size_t plane, float3* u,
                bool write_u_all, const SpreadLists& sl, const float3* Fl, hipStream_t s, hipEvent_t ev0,
@@ -550,12 +564,15 @@ void step_impl(const LatticeGeom& g, const LbmParams& p, const void* fin, void* 
   const T* in = static_cast<const T*>(fin);
   T* out = static_cast<T*>(fout);
   const int wu = write_u_all ? 1 : 0;
+  const bool b32 = fits_b32(plane, sizeof(T));
+  auto go = [&](auto kernel) {
+    launch_maybe_timed(kernel, blocks_for(g.N), s, ev0, ev1, g, p, in, out, (unsigned)plane, u, wu, sl, Fl,
+                       const_force);
+  };
   if (nt)
-    launch_maybe_timed(k_lbm_step<S, true>, blocks_for(g.N), s, ev0, ev1, g, p, in, out, (unsigned)plane, u, wu, sl,
-                       Fl, const_force);
+    b32 ? go(k_lbm_step<S, true, true>) : go(k_lbm_step<S, true, false>);
   else
-    launch_maybe_timed(k_lbm_step<S, false>, blocks_for(g.N), s, ev0, ev1, g, p, in, out, (unsigned)plane, u, wu,
-                       sl, Fl, const_force);
+    b32 ? go(k_lbm_step<S, false, true>) : go(k_lbm_step<S, false, false>);
 }
 
 }  // namespace
@@ -609,6 +626,7 @@ void step_batch_impl(const BatchTables& t, int sel, bool write_u_all, const LbmL
 void launch_lbm_step_batch(int storage, const BatchTables& t, int sel, bool write_u_all, const LbmLaunch& shape,
                            hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
   if (t.nblk == 0) return;
+  // (members' distribution buffers are below 4 GiB: checked when they join)
   if (shape.run != 0 && shape.run != 4) throw std::invalid_argument("XCD run length must be 0 or 4");
   GLSM_STORAGE_DISPATCH(storage, (step_batch_impl<S>(t, sel, write_u_all, shape, s, ev0, ev1)));
   GLSM_LAUNCH_CHECK();
