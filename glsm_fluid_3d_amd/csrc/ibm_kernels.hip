@@ -180,14 +180,13 @@ __device__ __forceinline__ void repulsion_wave(const MarkerArrays& m, float delt
 // k * M + l, so one thread per marker reads them coalesced.
 __device__ __forceinline__ size_t tab(const MarkerArrays& m, int l, int k) { return (size_t)k * m.M + l; }
 
-// Ul / Fdrag from the 64 stencil sums (shared by both interpolation forms)
-__device__ __forceinline__ void interp_finish(const MarkerArrays& m, const SpreadLists& sl, float beta_eff, int l,
+// Ul / Fdrag from the 64 stencil sums, the weight sum ws and the marker
+// velocity vl (shared by both interpolation forms)
+__device__ __forceinline__ void interp_finish(const MarkerArrays& m, float beta_eff, int l, float ws, float3 vl,
                                               float ux, float uy, float uz) {
-  const float ws = sl.wsum[l];
   float3 ul = make_float3(0.f, 0.f, 0.f);
   if (ws > 0.0f) ul = make_float3(ux / ws, uy / ws, uz / ws);
   m.Ul[l] = ul;
-  const float3 vl = m.Vl[l];
   m.Fdrag[l] = make_float3(beta_eff * (vl.x - ul.x), beta_eff * (vl.y - ul.y), beta_eff * (vl.z - ul.z));
 }
 
@@ -195,15 +194,19 @@ __device__ __forceinline__ void interp_finish(const MarkerArrays& m, const Sprea
 // summed in lane order (for small marker counts: more parallelism).
 __device__ __forceinline__ void interp_wave(const MarkerArrays& m, const SpreadLists& sl,
                                             const float3* __restrict__ u, float beta_eff, int l, int lane) {
+  // the marker's weight sum and velocity are loaded ahead of the stencil
+  const float ws = gmem(sl.wsum)[l];
+  const GLSM_GLOBAL float* vp = gmem(reinterpret_cast<const float*>(m.Vl)) + 3 * (size_t)l;
+  const float3 vl = make_float3(vp[0], vp[1], vp[2]);
   const size_t k = tab(m, l, lane);
-  const int cell = sl.ctab[k];
+  const int cell = gmem(sl.ctab)[k];
   float px = 0.f, py = 0.f, pz = 0.f;
   if (cell >= 0) {
-    const float w = sl.rtab[k];
-    const float3 uc = u[cell];
-    px = uc.x * w;
-    py = uc.y * w;
-    pz = uc.z * w;
+    const float w = gmem(sl.rtab)[k];
+    const GLSM_GLOBAL float* up = gmem(reinterpret_cast<const float*>(u)) + 3 * (size_t)cell;
+    px = up[0] * w;
+    py = up[1] * w;
+    pz = up[2] * w;
   }
   float ux = 0.f, uy = 0.f, uz = 0.f;
 #pragma unroll
@@ -212,7 +215,7 @@ __device__ __forceinline__ void interp_wave(const MarkerArrays& m, const SpreadL
     uy += lane_read(py, j);
     uz += lane_read(pz, j);
   }
-  if (lane == 0) interp_finish(m, sl, beta_eff, l, ux, uy, uz);
+  if (lane == 0) interp_finish(m, beta_eff, l, ws, vl, ux, uy, uz);
 }
 
 // One thread per (marker, velocity component) summing the 64 stencil cells
@@ -249,12 +252,12 @@ __device__ __forceinline__ float stencil_sum(const MarkerArrays& m, const Spread
 
 __device__ __forceinline__ void interp_component(const MarkerArrays& m, const SpreadLists& sl,
                                                  const float3* __restrict__ u, float beta_eff, int l, int comp) {
+  const float ws = gmem(sl.wsum)[l];  // loaded ahead of the stencil sums
+  const float vl = gmem(reinterpret_cast<const float*>(m.Vl))[3 * (size_t)l + comp];
   const float acc = stencil_sum(m, sl, reinterpret_cast<const float*>(u), 3, comp, l);
-  const float ws = sl.wsum[l];
   const float ul = ws > 0.0f ? acc / ws : 0.f;
-  const float vl = reinterpret_cast<const float*>(m.Vl)[3 * (size_t)l + comp];
-  reinterpret_cast<float*>(m.Ul)[3 * (size_t)l + comp] = ul;
-  reinterpret_cast<float*>(m.Fdrag)[3 * (size_t)l + comp] = beta_eff * (vl - ul);
+  gmem(reinterpret_cast<float*>(m.Ul))[3 * (size_t)l + comp] = ul;
+  gmem(reinterpret_cast<float*>(m.Fdrag))[3 * (size_t)l + comp] = beta_eff * (vl - ul);
 }
 
 // Dl = sum(c*phi)/sum(phi), Sl = Cl - Dl (k_ibm_sample_concentration +
