@@ -109,6 +109,47 @@ def run_batch(G, storage, steps, warmup, count=64, tune=True):
     return out
 
 
+def run_slabs(G, storage="f32", parts=8, steps=40):
+    """One GPU's share of a z-decomposed lattice: the 201^3 walled lattice
+    split into `parts` z-slabs (25-26 planes + 2 halo planes each) stepped as
+    a chain on this one GPU.  Reports the stream/collide launch at the slab's
+    shape, the pack -> transfer -> unpack phase (transfer = a device copy
+    standing in for the xGMI send/recv) and the face bytes one slab sends
+    per sub-step and direction."""
+    gels, c, f = cube201_four_gels(G, storage)
+    chain = [G.Slab(f, z0, nz) for z0, nz in G.split_z(f.grid[2], parts)]
+    try:
+        for sl in chain:
+            sl.load(f)
+        G.slab_chain_step(chain, 5)
+        G.synchronize()
+        t0 = time.perf_counter()
+        G.slab_chain_step(chain, steps)
+        G.synchronize()
+        dt = time.perf_counter() - t0
+        for sl in chain:
+            sl.enable_timing(True)
+        G.slab_chain_step(chain, 10)
+        tim = [sl.timing() for sl in chain]
+        face = chain[0].face_bytes()
+        full = [sl for sl in chain if sl.nz == max(x.nz for x in chain)]
+        lbm_ms = float(np.median([tim[chain.index(sl)][0] for sl in full]))
+        xch_ms = float(np.median([t[1] for t in tim]))
+        nz = full[0].nz
+        own = f.grid[0] * f.grid[1] * nz
+        return {"slabs": parts, "slab_shape": f"{f.grid[0]}x{f.grid[1]}x{nz} (+2 halo planes)",
+                "chain_MLUPS_one_gpu": round(f.N * steps / dt / 1e6, 1),
+                "slab_lbm_ms": round(lbm_ms, 5),
+                "slab_lbm_GBs_own_planes": round(own * 2 * 19 * G.STORAGE_BYTES[storage] / (lbm_ms * 1e-3) / 1e9, 1),
+                "exchange_ms_chain_phase": round(xch_ms, 5), "face_bytes": face,
+                "xgmi_us_per_face_at_153GBs": round(face / 153e9 * 1e6, 2)}
+    finally:
+        for sl in chain:
+            sl.close()
+        for x in gels + [c, f]:
+            x.close()
+
+
 def run_serve(G, seconds, outputs=True, seed=4):
     """The serving workload for `seconds` of wall time: Poisson arrivals at
     20 jobs/s, edge 41-161, 1-6 gels of 6^3, horizons 1000-20000 steps, FP16
@@ -204,7 +245,7 @@ def main():
                     help="IBM interpolation form: 0 by marker count, 1 wavefront per marker, 2 thread per marker")
     ap.add_argument("--workload", default="all",
                     choices=["all", "101", "201-f32", "201-f16", "201-bf16", "copy", "batch-f32", "batch-f16",
-                             "serve"],
+                             "serve", "slab"],
                     help="profile one workload alone (no CPU baseline); 'copy' = the 1.23 GB calibration copy")
     args = ap.parse_args()
     if args.workload != "all":
@@ -227,6 +268,11 @@ def main():
     if args.workload == "serve":
         r = run_serve(G, args.serve_seconds)
         r["workload"] = f"Poisson 20 jobs/s for {args.serve_seconds:.0f} s, f16, async .dat output"
+        print(json.dumps(r), flush=True)
+        return
+    if args.workload == "slab":
+        r = run_slabs(G, args.storage)
+        r["workload"] = f"201^3 walled lattice in 8 z-slabs on one GPU, {args.storage}"
         print(json.dumps(r), flush=True)
         return
     if args.workload.startswith("batch"):

@@ -99,6 +99,14 @@ SIGNATURES = {
     "glsm_batch_autotune": (_i, [_vp, _i, C.POINTER(_d)]),
     "glsm_batch_enable_lbm_timing": (_i, [_vp, _i]),
     "glsm_batch_lbm_timing": (_i, [_vp, C.POINTER(_d), C.POINTER(_i)]),
+    "glsm_slab_create": (_i, [_vp, _i, _i, C.POINTER(_vp)]),
+    "glsm_slab_destroy": (_i, [_vp]),
+    "glsm_slab_load": (_i, [_vp, _vp]),
+    "glsm_slab_store": (_i, [_vp, _vp]),
+    "glsm_slab_chain_step": (_i, [C.POINTER(_vp), _i, _i, _i]),
+    "glsm_slab_face_bytes": (_i, [_vp, C.POINTER(_sz)]),
+    "glsm_slab_enable_timing": (_i, [_vp, _i]),
+    "glsm_slab_timing": (_i, [_vp, C.POINTER(_d), C.POINTER(_d)]),
     "glsm_output_create": (_i, [_i, C.POINTER(_vp)]),
     "glsm_output_destroy": (_i, [_vp]),
     "glsm_output_gel": (_i, [_vp, _vp, _i, C.c_char_p]),
@@ -460,6 +468,59 @@ class Batch:
         ms, n = C.c_double(), C.c_int()
         _check(load().glsm_batch_lbm_timing(self.h, C.byref(ms), C.byref(n)), "batch lbm_timing")
         return ms.value, n.value
+
+
+class Slab:
+    """Planes [z0, z0 + nz) of `like`'s lattice with one halo plane on each
+    side (include/slab.h): one GPU's share of a z-decomposed lattice."""
+
+    def __init__(self, like: "Fluid", z0: int, nz: int):
+        h = C.c_void_p()
+        _check(load().glsm_slab_create(like.h, int(z0), int(nz), C.byref(h)), "glsm_slab_create")
+        self.h = h
+        self.z0, self.nz = int(z0), int(nz)
+
+    def close(self):
+        if self.h:
+            _check(load().glsm_slab_destroy(self.h), "glsm_slab_destroy")
+            self.h = None
+
+    def load(self, src: "Fluid"):
+        _check(load().glsm_slab_load(self.h, src.h), "glsm_slab_load")
+
+    def store(self, dst: "Fluid"):
+        _check(load().glsm_slab_store(self.h, dst.h), "glsm_slab_store")
+
+    def face_bytes(self) -> int:
+        n = C.c_size_t()
+        _check(load().glsm_slab_face_bytes(self.h, C.byref(n)), "glsm_slab_face_bytes")
+        return n.value
+
+    def enable_timing(self, on=True):
+        _check(load().glsm_slab_enable_timing(self.h, 1 if on else 0), "glsm_slab_enable_timing")
+
+    def timing(self):
+        """(stream/collide ms, pack -> transfer -> unpack ms), mean per sub-step."""
+        a, b = C.c_double(), C.c_double()
+        _check(load().glsm_slab_timing(self.h, C.byref(a), C.byref(b)), "glsm_slab_timing")
+        return a.value, b.value
+
+
+def slab_chain_step(chain, steps=1, write_u=False):
+    """`steps` sub-steps of a chain of slabs covering the lattice in z order."""
+    arr = (C.c_void_p * len(chain))(*[s.h for s in chain])
+    _check(load().glsm_slab_chain_step(arr, len(chain), int(steps), 1 if write_u else 0), "glsm_slab_chain_step")
+
+
+def split_z(nz_total: int, parts: int):
+    """(z0, nz) of `parts` contiguous slabs, sizes differing by at most one."""
+    base, extra = divmod(nz_total, parts)
+    out, z = [], 0
+    for k in range(parts):
+        n = base + (1 if k < extra else 0)
+        out.append((z, n))
+        z += n
+    return out
 
 
 class OutputStream:

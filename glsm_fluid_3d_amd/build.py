@@ -38,8 +38,8 @@ NO_CONTRACT = {"gel_kernels.hip", "ibm_kernels.hip"}
 # the same collision code then rounds identically in every kernel it is
 # inlined into (single lattice, batched, persistent)
 CONTRACT_ON = {"lbm_kernels.hip"}
-SOURCES = ["gel_kernels.hip", "ibm_kernels.hip", "lbm_kernels.hip",
-           "runtime.cpp", "gel.cpp", "coupling.cpp", "fluid.cpp", "batch.cpp", "dat_format.cpp",
+SOURCES = ["gel_kernels.hip", "ibm_kernels.hip", "lbm_kernels.hip", "slab_kernels.hip",
+           "runtime.cpp", "gel.cpp", "coupling.cpp", "fluid.cpp", "batch.cpp", "slab.cpp", "dat_format.cpp",
            "output.cpp", "capi.cpp"]
 
 

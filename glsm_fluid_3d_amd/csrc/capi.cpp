@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "batch.h"
+#include "slab.h"
 #include "output.h"
 #include "coupling.h"
 #include "fluid.h"
@@ -826,6 +827,83 @@ int glsm_device_memory(size_t* free_bytes, size_t* total_bytes) {
 int glsm_set_interp_mode(int mode) {
   return guarded([&] {
     glsm::set_interp_mode(mode);
+    return GLSM_OK;
+  });
+}
+
+// ---------------------------------------------------------------- slabs
+struct glsm_slab {
+  Slab* s = nullptr;
+};
+namespace {
+Slab* SL(glsm_slab* s) {
+  REQUIRE(s && s->s, "null slab handle");
+  return s->s;
+}
+}  // namespace
+
+int glsm_slab_create(glsm_fluid* like, int z0, int nz, glsm_slab** out) {
+  return guarded([&] {
+    REQUIRE(out, "null output pointer");
+    Fluid* f = F(like);
+    auto* h = new glsm_slab;
+    try {
+      h->s = new Slab(*f, z0, nz);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+    return GLSM_OK;
+  });
+}
+int glsm_slab_destroy(glsm_slab* s) {
+  return guarded([&] {
+    delete SL(s);
+    delete s;
+    return GLSM_OK;
+  });
+}
+int glsm_slab_load(glsm_slab* s, glsm_fluid* src) {
+  return guarded([&] {
+    SL(s)->load(*F(src));
+    return GLSM_OK;
+  });
+}
+int glsm_slab_store(glsm_slab* s, glsm_fluid* dst) {
+  return guarded([&] {
+    SL(s)->store(*F(dst));
+    return GLSM_OK;
+  });
+}
+int glsm_slab_chain_step(glsm_slab* const* chain, int n, int steps, int write_u) {
+  return guarded([&] {
+    REQUIRE(chain && n > 0, "empty slab chain");
+    REQUIRE(steps >= 0, "negative step count");
+    std::vector<Slab*> v;
+    for (int i = 0; i < n; ++i) v.push_back(SL(chain[i]));
+    for (int k = 0; k < steps; ++k) step_slab_chain(v, write_u != 0);
+    return GLSM_OK;
+  });
+}
+int glsm_slab_face_bytes(glsm_slab* s, size_t* bytes) {
+  return guarded([&] {
+    REQUIRE(bytes, "null output pointer");
+    *bytes = SL(s)->faceBytes();
+    return GLSM_OK;
+  });
+}
+int glsm_slab_enable_timing(glsm_slab* s, int on) {
+  return guarded([&] {
+    SL(s)->enableTiming(on != 0);
+    return GLSM_OK;
+  });
+}
+int glsm_slab_timing(glsm_slab* s, double* lbm_ms, double* exchange_ms) {
+  return guarded([&] {
+    REQUIRE(lbm_ms && exchange_ms, "null output pointer");
+    *lbm_ms = SL(s)->lbmKernelMs();
+    *exchange_ms = SL(s)->exchangeKernelMs();
     return GLSM_OK;
   });
 }

@@ -27,6 +27,8 @@ glsm::LatticeGeom geom(const Fluid& f) {
   g.perX = f.h_fp->perX;
   g.perY = f.h_fp->perY;
   g.perZ = f.h_fp->perZ;
+  g.zlo = 0;
+  g.zhi = g.Nz - 1;
   g.h = f.h_fp->h;
   g.rNx = 1.0 / g.Nx;
   g.rNxNy = 1.0 / ((double)g.Nx * g.Ny);

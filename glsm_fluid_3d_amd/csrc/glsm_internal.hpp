@@ -155,6 +155,10 @@ void launch_repulsion(const MarkerArrays& m, float delta, float gamma, hipStream
 struct LatticeGeom {
   int Nx, Ny, Nz, N;
   int perX, perY, perZ;
+  // planes z in (zlo, zhi] have a lower neighbour plane and [zlo, zhi) an
+  // upper one (a whole lattice: 0 and Nz - 1; a z-slab with halo planes:
+  // its halos, or its first / last own plane where a wall bounds it)
+  int zlo, zhi;
   float h;
   double rNx, rNxNy;  // 1/Nx, 1/(Nx*Ny) for exact index decoding
 };
@@ -292,5 +296,20 @@ void launch_lbm_step_batch(int storage, const BatchTables& t, int sel, bool writ
                            hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 // iters[i] += 1 for the n members
 void launch_advance_iters(int* iters, int n, hipStream_t s);
+
+// ---- z-slab decomposition (slab_kernels.hip): copies of the stored
+// representation, `elem` = 2 or 4 bytes per value ----
+// slab planes 0..nz+1 <- lattice planes z0-1..z0+nz (wrapped if perZ)
+void launch_slab_gather(size_t elem, const void* src, size_t splane, void* dst, size_t dplane, size_t nxy, int z0,
+                        int nz, int Nz, int perZ, hipStream_t s);
+// lattice planes z0..z0+nz-1 <- slab planes 1..nz
+void launch_slab_scatter(size_t elem, const void* src, size_t splane, void* dst, size_t dplane, size_t nxy, int z0,
+                         int nz, hipStream_t s);
+// face buffer (5 x nxy) <-> the c_z = +1 (which = 0) or c_z = -1 (which = 1)
+// populations of slab plane z
+void launch_face_pack(size_t elem, const void* f, size_t plane, size_t nxy, int z, int which, void* out,
+                      hipStream_t s);
+void launch_face_unpack(size_t elem, const void* in, void* f, size_t plane, size_t nxy, int z, int which,
+                        hipStream_t s);
 
 }  // namespace glsm

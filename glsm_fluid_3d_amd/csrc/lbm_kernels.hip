@@ -158,7 +158,7 @@ struct PullAddr {
   bool kxp, kxm, kyp, kym, kzp, kzm;
   __device__ PullAddr(const LatticeGeom& g, int x, int y, int z) {
     const int sx = 1, sy = g.Nx, sz = g.Nx * g.Ny;
-    const bool xl = x > 0, xh = x < g.Nx - 1, yl = y > 0, yh = y < g.Ny - 1, zl = z > 0, zh = z < g.Nz - 1;
+    const bool xl = x > 0, xh = x < g.Nx - 1, yl = y > 0, yh = y < g.Ny - 1, zl = z > g.zlo, zh = z < g.zhi;
     kxp = xl || g.perX; oxp = xl ? -sx : (g.Nx - 1) * sx;
     kxm = xh || g.perX; oxm = xh ? sx : -(g.Nx - 1) * sx;
     kyp = yl || g.perY; oyp = yl ? -sy : (g.Ny - 1) * sy;

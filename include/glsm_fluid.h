@@ -241,6 +241,31 @@ int glsm_output_status(glsm_output* o, long* pending, long* completed, char* err
 /* attach (or detach with NULL) the stream to a batch */
 int glsm_batch_set_output(glsm_batch* b, glsm_output* o);
 
+/* ---- z-slab decomposition (include/slab.h; no reference counterpart: the
+   reference runs one lattice on one device).  A slab holds planes
+   [z0, z0 + nz) of a fluid's lattice plus one halo plane on each side and
+   advances them with the same stream/collide kernel (F_const only, no
+   marker force).  A chain of slabs covering the lattice in z order steps
+   together: each slab steps, packs the five populations crossing each face
+   into a face buffer, receives its neighbours' buffers (a device copy here;
+   an RCCL send/recv or peer copy across GPUs) and unpacks them into its
+   halos.  A chain reproduces the whole lattice's stream/collide sub-steps
+   bit for bit. ---- */
+typedef struct glsm_slab glsm_slab;
+int glsm_slab_create(glsm_fluid* like, int z0, int nz, glsm_slab** out);
+int glsm_slab_destroy(glsm_slab* s);
+/* own planes and halos <- src's stored state; own planes -> dst's */
+int glsm_slab_load(glsm_slab* s, glsm_fluid* src);
+int glsm_slab_store(glsm_slab* s, glsm_fluid* dst);
+/* `steps` sub-steps of the chain (slabs in z order covering the lattice) */
+int glsm_slab_chain_step(glsm_slab* const* chain, int n, int steps, int write_u);
+/* bytes of one face buffer (5 x Nx x Ny stored values) */
+int glsm_slab_face_bytes(glsm_slab* s, size_t* bytes);
+/* event timing (ms, mean per sub-step since enabled): the slab's
+   stream/collide launch, and its pack -> transfer -> unpack phase */
+int glsm_slab_enable_timing(glsm_slab* s, int on);
+int glsm_slab_timing(glsm_slab* s, double* lbm_ms, double* exchange_ms);
+
 /* device memory of the current device, bytes (hipMemGetInfo) */
 int glsm_device_memory(size_t* free_bytes, size_t* total_bytes);
 
