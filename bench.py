@@ -150,22 +150,23 @@ def run_slabs(G, storage="f32", parts=8, steps=40):
             x.close()
 
 
-def run_serve(G, seconds, outputs=True, seed=4):
+def run_serve(G, seconds, outputs=True, seed=4, rate=20.0, drain=False):
     """The serving workload for `seconds` of wall time: Poisson arrivals at
-    20 jobs/s, edge 41-161, 1-6 gels of 6^3, horizons 1000-20000 steps, FP16
-    storage, .dat output streamed asynchronously (into a scratch directory
-    removed afterwards)."""
+    `rate` jobs/s (20 by default), edge 41-161, 1-6 gels of 6^3, horizons
+    1000-20000 steps, FP16 storage, .dat output streamed asynchronously (into
+    a scratch directory removed afterwards).  With `drain`, arrivals stop
+    after `seconds` and the server runs until every job has completed."""
     import shutil
     import tempfile
 
     from glsm_fluid_3d_amd import serving as S
     from glsm_fluid_3d_amd import workloads as W
 
-    jobs = W.poisson_jobs(seed=seed, duration=seconds)
+    jobs = W.poisson_jobs(seed=seed, rate=rate, duration=seconds)
     root = tempfile.mkdtemp(prefix="glsm_serve_") if outputs else None
     srv = S.Server(G, storage="f16", out_root=root, keep_output=False, writers=8)
     try:
-        stats = srv.run(jobs, duration=seconds, progress_every=30.0)
+        stats = srv.run(jobs, duration=None if drain else seconds, progress_every=30.0)
     finally:
         srv.close()
         if root:
@@ -241,6 +242,9 @@ def main():
     ap.add_argument("--extra-steps", type=int, default=400)
     ap.add_argument("--batch-steps", type=int, default=300)
     ap.add_argument("--serve-seconds", type=float, default=45.0)
+    ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrival rate of the serving workload")
+    ap.add_argument("--serve-drain", action="store_true",
+                    help="stop arrivals after --serve-seconds and run until every job has completed")
     ap.add_argument("--interp-mode", type=int, default=0, choices=[0, 1, 2],
                     help="IBM interpolation form: 0 by marker count, 1 wavefront per marker, 2 thread per marker")
     ap.add_argument("--workload", default="all",
@@ -266,8 +270,9 @@ def main():
                           "GBs": round(G.copy_bandwidth(nbytes, max(1, args.steps)), 1)}), flush=True)
         return
     if args.workload == "serve":
-        r = run_serve(G, args.serve_seconds)
-        r["workload"] = f"Poisson 20 jobs/s for {args.serve_seconds:.0f} s, f16, async .dat output"
+        r = run_serve(G, args.serve_seconds, rate=args.serve_rate, drain=args.serve_drain)
+        r["workload"] = (f"Poisson {args.serve_rate:g} jobs/s for {args.serve_seconds:.0f} s"
+                         f"{' then drained' if args.serve_drain else ''}, f16, async .dat output")
         print(json.dumps(r), flush=True)
         return
     if args.workload == "slab":
