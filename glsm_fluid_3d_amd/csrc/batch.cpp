@@ -132,7 +132,8 @@ void Batch::useGraph(bool on) { use_graph_ = on; }
 Batch::Launch Batch::launch() const { return launch_; }
 
 void Batch::setLaunch(const Launch& l) {
-  if (l.run != 0 && l.run != 4) throw std::invalid_argument("Batch::setLaunch: run must be 0 or 4");
+  if (l.run != 0 && l.run != 4 && l.run != 16 && l.run != 64)
+    throw std::invalid_argument("Batch::setLaunch: run must be 0, 4, 16 or 64");
   if (l.grid < 0) throw std::invalid_argument("Batch::setLaunch: negative grid");
   launch_ = l;
   launch_set_ = true;
@@ -341,7 +342,7 @@ Batch::Launch Batch::autotune(int reps, double* best_ms) {
   }
   std::vector<Launch> cand;
   for (int nt = 0; nt < 2; ++nt)
-    for (int run : {0, 4})
+    for (int run : {0, 4, 16, 64})
       for (int g : {0, 2, 4, 8, 16}) cand.push_back(Launch{nt, run, g ? g * cus : 0});
   hipEvent_t e0, e1;
   GLSM_HIP(hipEventCreate(&e0));

@@ -613,21 +613,26 @@ void step_batch_impl(const BatchTables& t, int sel, bool write_u_all, const LbmL
                      hipEvent_t ev0, hipEvent_t ev1) {
   const int grid = sh.grid > 0 && sh.grid < t.nblk ? sh.grid : t.nblk;
   const int wu = write_u_all ? 1 : 0;
-  if (sh.nt && sh.run)
-    launch_maybe_timed(kb_lbm_step<S, true, 4>, grid, s, ev0, ev1, t, sel, wu);
-  else if (sh.nt)
-    launch_maybe_timed(kb_lbm_step<S, true, 0>, grid, s, ev0, ev1, t, sel, wu);
-  else if (sh.run)
-    launch_maybe_timed(kb_lbm_step<S, false, 4>, grid, s, ev0, ev1, t, sel, wu);
-  else
-    launch_maybe_timed(kb_lbm_step<S, false, 0>, grid, s, ev0, ev1, t, sel, wu);
+  auto go = [&](auto kernel) { launch_maybe_timed(kernel, grid, s, ev0, ev1, t, sel, wu); };
+  switch (sh.run * 2 + (sh.nt ? 1 : 0)) {
+    case 0: go(kb_lbm_step<S, false, 0>); break;
+    case 1: go(kb_lbm_step<S, true, 0>); break;
+    case 8: go(kb_lbm_step<S, false, 4>); break;
+    case 9: go(kb_lbm_step<S, true, 4>); break;
+    case 32: go(kb_lbm_step<S, false, 16>); break;
+    case 33: go(kb_lbm_step<S, true, 16>); break;
+    case 128: go(kb_lbm_step<S, false, 64>); break;
+    case 129: go(kb_lbm_step<S, true, 64>); break;
+    default: throw std::invalid_argument("XCD run length must be 0, 4, 16 or 64");
+  }
 }
 
 void launch_lbm_step_batch(int storage, const BatchTables& t, int sel, bool write_u_all, const LbmLaunch& shape,
                            hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
   if (t.nblk == 0) return;
   // (members' distribution buffers are below 4 GiB: checked when they join)
-  if (shape.run != 0 && shape.run != 4) throw std::invalid_argument("XCD run length must be 0 or 4");
+  if (shape.run != 0 && shape.run != 4 && shape.run != 16 && shape.run != 64)
+    throw std::invalid_argument("XCD run length must be 0, 4, 16 or 64");
   GLSM_STORAGE_DISPATCH(storage, (step_batch_impl<S>(t, sel, write_u_all, shape, s, ev0, ev1)));
   GLSM_LAUNCH_CHECK();
 }

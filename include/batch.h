@@ -64,7 +64,7 @@ class Batch {
   // Launch shape of the batched stream/collide kernel.
   struct Launch {
     int nt;    // non-temporal stores
-    int run;   // XCD run length: 0 (one contiguous run per XCD) or 4
+    int run;   // XCD run length: 0 (one contiguous run per XCD), 4, 16 or 64
     int grid;  // work-groups; 0 = one per 256-cell block, else persistent
   };
   Launch launch() const;

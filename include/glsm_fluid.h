@@ -214,7 +214,7 @@ int glsm_batch_iteration(glsm_batch* b, int id);
 int glsm_batch_step(glsm_batch* b, int nsteps);
 int glsm_batch_use_graph(glsm_batch* b, int on);
 /* stream/collide launch shape: nt = non-temporal stores, run = XCD run length
-   (0 or 4), grid = work-groups (0 = one per 256-cell block, else
+   (0, 4, 16 or 64), grid = work-groups (0 = one per 256-cell block, else
    persistent) */
 int glsm_batch_set_launch(glsm_batch* b, int nt, int run, int grid);
 int glsm_batch_get_launch(glsm_batch* b, int* nt, int* run, int* grid);

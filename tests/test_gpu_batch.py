@@ -92,7 +92,7 @@ def test_batch_members_equal_solo_runs(glsm, graph):
 def test_batch_fp16_and_every_launch_shape(glsm):
     G = glsm
     ref = [solo(G, s, 0, 6, "f16") for s in SPECS]
-    shapes = [(nt, run, grid) for nt in (0, 1) for run in (0, 4) for grid in (0, 3, 256)]
+    shapes = [(nt, run, grid) for nt in (0, 1) for run in (0, 4, 16, 64) for grid in (0, 3, 256)]
     for shape in shapes:
         b = G.Batch("f16")
         insts = [build(G, s, "f16") for s in SPECS]
@@ -120,7 +120,7 @@ def test_batch_autotune_keeps_state_and_results(glsm):
         b.step(3)
         before = [state(inst) for inst in insts]
         shape, ms = b.autotune(reps=2)
-        assert ms > 0 and shape["run"] in (0, 4)
+        assert ms > 0 and shape["run"] in (0, 4, 16, 64)
         for k, inst in enumerate(insts):
             assert_same(state(inst), before[k], f"autotune changed member {k}")
         b.step(5)
@@ -173,7 +173,7 @@ def test_batch_rejects_bad_members(glsm):
             b.add(*inst)
         with pytest.raises(G.GlsmError, match="no member"):
             b.remove(12345)
-        with pytest.raises(G.GlsmError, match="run must be 0 or 4"):
+        with pytest.raises(G.GlsmError, match="run must be 0, 4, 16 or 64"):
             b.set_launch(0, 3, 0)
     finally:
         b.close()
