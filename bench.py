@@ -52,6 +52,20 @@ def cube101_two_gels(mod, storage=None, prec=None, seed=1):
     return gels, c, f
 
 
+def gel_array_3x3(mod, storage=None):
+    """The 3 x 3 gel array of sim_array.cpp:31-72: a 140 x 140 x 10 box
+    (281 x 281 x 21 nodes), nine 20 x 20 x 4-element gels at x, y in
+    {30, 70, 110}, z = 7 -- 10,098 markers, whose O(M^2) repulsion is the
+    large-M case of the coupler."""
+    gels = [mod.Gel((20, 20, 4), (float(x), float(y), 7.0), gel_id=k + 1)
+            for k, (y, x) in enumerate((y, x) for y in (30, 70, 110) for x in (30, 70, 110))]
+    c = mod.Coupler(gels)
+    f = mod.Fluid((140, 140, 10), c)
+    if storage:
+        f.set_storage(storage)
+    return gels, c, f
+
+
 def four_gel_centres(seed=2, box=100.0, n=8, count=4):
     """Seeded uniform centres of n^3-element gels, rejected when their boxes
     come closer than one element (the 201^3 precision workload)."""
@@ -249,7 +263,7 @@ def main():
                     help="IBM interpolation form: 0 by marker count, 1 wavefront per marker, 2 thread per marker")
     ap.add_argument("--workload", default="all",
                     choices=["all", "101", "201-f32", "201-f16", "201-bf16", "copy", "batch-f32", "batch-f16",
-                             "serve", "slab"],
+                             "serve", "slab", "array"],
                     help="profile one workload alone (no CPU baseline); 'copy' = the 1.23 GB calibration copy")
     args = ap.parse_args()
     if args.workload != "all":
@@ -274,6 +288,16 @@ def main():
         r["workload"] = (f"Poisson {args.serve_rate:g} jobs/s for {args.serve_seconds:.0f} s"
                          f"{' then drained' if args.serve_drain else ''}, f16, async .dat output")
         print(json.dumps(r), flush=True)
+        return
+    if args.workload == "array":
+        gels, c, f = gel_array_3x3(G, args.storage)
+        dt, ms, n = run_gpu(G, gels, c, f, args.steps, args.warmup)
+        print(json.dumps({"workload": "sim_array 3x3 gel array: 281x281x21 nodes, nine 20x20x4 gels",
+                          "markers": c.M, "value": round(f.N * f.nsub * args.steps / dt / 1e6, 1), "unit": "MLUPS",
+                          "steps": args.steps, "ms_per_step": round(dt / args.steps * 1e3, 4),
+                          "roofline": roofline(f.N, 2 * 19 * G.STORAGE_BYTES[args.storage], ms)}), flush=True)
+        for x in gels + [c, f]:
+            x.close()
         return
     if args.workload == "slab":
         r = run_slabs(G, args.storage)
