@@ -96,7 +96,8 @@ int Batch::add(std::vector<Gel*>& gels, Coupler* coupler, Fluid* fluid, int iter
   if (fluid->storage() != storage_ || (storage_ == Fluid::F16_DEV && fluid->storageScaleExp() != scale_exp_))
     fluid->setStorage(storage_, scale_exp_);
   GLSM_HIP(hipStreamSynchronize(glsm::stream()));
-  members_.push_back(Member{next_id_, &gels, coupler, fluid, iter, write_files, out_dir});
+  members_.push_back(Member{next_id_, &gels, coupler, fluid, iter, write_files, out_dir, {nullptr, nullptr},
+                            {nullptr, nullptr}});
   dirty_ = true;
   return next_id_++;
 }
@@ -154,6 +155,12 @@ void Batch::rebuild() {
   for (size_t i = 0; i < members_.size(); ++i) {
     Member& m = members_[i];
     Fluid& f = *m.fluid;
+    if (f.storage() != storage_ || (storage_ == Fluid::F16_DEV && f.storageScaleExp() != scale_exp_))
+      throw std::invalid_argument("Batch: a member's storage changed; remove it and add it again");
+    m.ftab[pf_] = f.d_f;
+    m.ftab[pf_ ^ 1] = f.d_fnext;
+    m.ctab[pc_] = f.d_c1;
+    m.ctab[pc_ ^ 1] = f.d_c2;
     glsm::BatchInst b{};
     b.g = glsm::fluid_geom(f);
     b.p = glsm::fluid_lbm_params(f);
@@ -269,6 +276,8 @@ void Batch::enqueueStep(bool timed, bool lists) {
 void Batch::step(int nsteps) {
   if (nsteps < 0) throw std::invalid_argument("Batch::step: negative step count");
   if (members_.empty()) return;
+  for (const Member& m : members_)
+    if (m.fluid->d_f != m.ftab[pf_] || m.fluid->d_c1 != m.ctab[pc_]) dirty_ = true;
   if (dirty_) rebuild();
   hipStream_t s = glsm::stream();
   for (int n = 0; n < nsteps; ++n) {

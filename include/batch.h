@@ -88,6 +88,11 @@ class Batch {
     int iter;
     bool write_files;
     std::string out_dir;
+    // the fluid buffers bound into the device tables, by parity: a member
+    // stepped outside the batch, or given new buffers, no longer matches and
+    // the tables are rebuilt
+    void* ftab[2];
+    float* ctab[2];
   };
   void rebuild();
   bool listsNeeded() const;

@@ -163,6 +163,33 @@ def test_batch_members_join_and_leave(glsm):
             close(inst)
 
 
+def test_member_stepped_outside_the_batch(glsm):
+    """A member advanced by its own methods between batch steps (here one
+    extra LBM sub-step, which swaps its buffers) is picked up from its
+    current buffers, as in the same sequence run alone."""
+    G = glsm
+    spec = SPECS[1]
+    inst = build(G, spec)
+    for it in range(3):
+        G.system_step(inst[0], inst[1], inst[2], it)
+    inst[2].sub_step(0.5)
+    for it in range(3, 6):
+        G.system_step(inst[0], inst[1], inst[2], it)
+    ref = state(inst)
+    close(inst)
+    b = G.Batch("f32")
+    inst = build(G, spec)
+    try:
+        b.add(*inst)
+        b.step(3)
+        inst[2].sub_step(0.5)
+        b.step(3)
+        assert_same(state(inst), ref, "member stepped outside")
+    finally:
+        b.close()
+        close(inst)
+
+
 def test_batch_rejects_bad_members(glsm):
     G = glsm
     b = G.Batch("f32")
