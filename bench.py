@@ -328,7 +328,8 @@ def main():
     line = {
         "metric": "MLUPS (LBM lattice-node updates/s, full gel-fluid time step)",
         "value": round(mlups, 1), "unit": "MLUPS", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(dt / args.steps * 1e3, 4), "higher_is_better": True, "scaling": "weak",
+        "ms_per_step": round(dt / args.steps * 1e3, 4), "gel_steps_per_s": round(args.steps / dt, 1),
+        "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "101^3 D3Q19 + concentration, two 6x6x6-element gels at x=1/3,2/3 (sim.cpp "
                                "placement), walls, seeded chemistry perturbation, 25 LBM sub-steps per gel step",
