@@ -259,8 +259,9 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrival rate of the serving workload")
     ap.add_argument("--serve-drain", action="store_true",
                     help="stop arrivals after --serve-seconds and run until every job has completed")
-    ap.add_argument("--interp-mode", type=int, default=0, choices=[0, 1, 2],
-                    help="IBM interpolation form: 0 by marker count, 1 wavefront per marker, 2 thread per marker")
+    ap.add_argument("--interp-mode", type=int, default=0, choices=[0, 1, 2, 3],
+                    help="IBM interpolation form: 0 default (staged), 1 wavefront per marker, 2 thread per marker, "
+                         "3 staged in LDS")
     ap.add_argument("--workload", default="all",
                     choices=["all", "101", "201-f32", "201-f16", "201-bf16", "copy", "batch-f32", "batch-f16",
                              "serve", "slab", "array"],

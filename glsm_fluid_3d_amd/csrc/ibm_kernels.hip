@@ -260,6 +260,59 @@ __device__ __forceinline__ void interp_component(const MarkerArrays& m, const Sp
   gmem(reinterpret_cast<float*>(m.Fdrag))[3 * (size_t)l + comp] = beta_eff * (vl - ul);
 }
 
+// Staged form: a work-group takes kStageMarkers markers.  Phase 1, one
+// thread per (marker, 16-entry quarter of the stencil): the quarter's table
+// entries, then its 16 velocities, are loaded together and the 48 products
+// stored in LDS ([component][entry][marker], so both phases access
+// consecutive words across a wavefront).  Phase 2, one thread per (marker,
+// component): the 64 products added in stencil order from LDS.  Each marker
+// then waits for two dependent rounds of loads instead of eight, and every
+// table load is coalesced.  Same products, same order, same bits as the
+// other forms (cells outside the lattice add +-0 as in the thread form).
+constexpr int kStageMarkers = kThreads / 4;
+struct InterpStage {
+  float p[3][64][kStageMarkers];
+};
+__device__ __forceinline__ void interp_stage_load(const SpreadLists& sl, int M, const float3* __restrict__ u, int l,
+                                                  int quarter, int mi, InterpStage& st) {
+  const GLSM_GLOBAL int* ctab = gmem(sl.ctab);
+  const GLSM_GLOBAL float* rtab = gmem(sl.rtab);
+  const GLSM_GLOBAL float* gu = gmem(reinterpret_cast<const float*>(u));
+  const int k0 = 16 * quarter;
+  int cell[16];
+  float w[16], fx[16], fy[16], fz[16];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const size_t t = (size_t)(k0 + j) * M + l;
+    cell[j] = max(ctab[t], 0);
+    w[j] = rtab[t];
+  }
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    fx[j] = gu[3 * (size_t)cell[j]];
+    fy[j] = gu[3 * (size_t)cell[j] + 1];
+    fz[j] = gu[3 * (size_t)cell[j] + 2];
+  }
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    st.p[0][k0 + j][mi] = fx[j] * w[j];
+    st.p[1][k0 + j][mi] = fy[j] * w[j];
+    st.p[2][k0 + j][mi] = fz[j] * w[j];
+  }
+}
+__device__ __forceinline__ float interp_stage_sum(const InterpStage& st, int comp, int mi) {
+  float acc = 0.f;
+#pragma unroll
+  for (int k = 0; k < 64; ++k) acc += st.p[comp][k][mi];
+  return acc;
+}
+__device__ __forceinline__ void interp_stage_finish(const MarkerArrays& m, float beta_eff, int l, int comp, float ws,
+                                                    float vl, float acc) {
+  const float ul = ws > 0.0f ? acc / ws : 0.f;
+  gmem(reinterpret_cast<float*>(m.Ul))[3 * (size_t)l + comp] = ul;
+  gmem(reinterpret_cast<float*>(m.Fdrag))[3 * (size_t)l + comp] = beta_eff * (vl - ul);
+}
+
 // Dl = sum(c*phi)/sum(phi), Sl = Cl - Dl (k_ibm_sample_concentration +
 // k_robin_boundary, fluid_kernels.cu:350-387, 293-298)
 __device__ __forceinline__ void sample_finish(const MarkerArrays& m, const SpreadLists& sl, int l, float cl) {
@@ -505,6 +558,20 @@ __global__ void __launch_bounds__(kThreads) k_ibm_interp_t(MarkerArrays m, Sprea
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < 3 * m.M) interp_component(m, sl, u, beta_eff, t / 3, t % 3);
 }
+__global__ void __launch_bounds__(kThreads) k_ibm_interp_s(MarkerArrays m, SpreadLists sl, const float3* __restrict__ u,
+                                                           float beta_eff) {
+  __shared__ InterpStage st;
+  const int base = blockIdx.x * kStageMarkers, mi = threadIdx.x % kStageMarkers, part = threadIdx.x / kStageMarkers;
+  const int l = base + mi;
+  // phase 2's inputs (part < 3 = component): the marker's weight sum and
+  // velocity, loaded with phase 1's
+  const bool fin = part < 3 && l < m.M;
+  const float ws = fin ? gmem(sl.wsum)[l] : 0.f;
+  const float vl = fin ? gmem(reinterpret_cast<const float*>(m.Vl))[3 * (size_t)l + part] : 0.f;
+  if (l < m.M) interp_stage_load(sl, m.M, u, l, part, mi, st);
+  __syncthreads();
+  if (fin) interp_stage_finish(m, beta_eff, l, part, ws, vl, interp_stage_sum(st, part, mi));
+}
 __global__ void __launch_bounds__(kThreads) k_ibm_sample_t(MarkerArrays m, SpreadLists sl, const float* __restrict__ c) {
   const int l = blockIdx.x * blockDim.x + threadIdx.x;
   if (l < m.M) sample_thread(m, sl, c, l);
@@ -592,6 +659,21 @@ __global__ void __launch_bounds__(kThreads) kb_ibm_interp_t(BatchTables t) {
   const SpreadLists sl = I.sl;
   interp_component(m, sl, I.u, beta_eff_of(I, t.iters[t.mk_inst[gm]]), l, g3 % 3);
 }
+__global__ void __launch_bounds__(kThreads) kb_ibm_interp_s(BatchTables t) {
+  __shared__ InterpStage st;
+  const int base = blockIdx.x * kStageMarkers, mi = threadIdx.x % kStageMarkers, part = threadIdx.x / kStageMarkers;
+  const int gm = base + mi;
+  const bool live = gm < t.M;
+  int l = 0;
+  const BatchInst& I = marker_member(t, live ? gm : 0, l);
+  const bool fin = part < 3 && live;
+  const float beta_eff = fin ? beta_eff_of(I, t.iters[t.mk_inst[gm]]) : 0.f;
+  const float ws = fin ? gmem(I.sl.wsum)[l] : 0.f;
+  const float vl = fin ? gmem(reinterpret_cast<const float*>(I.m.Vl))[3 * (size_t)l + part] : 0.f;
+  if (live) interp_stage_load(I.sl, I.m.M, I.u, l, part, mi, st);
+  __syncthreads();
+  if (fin) interp_stage_finish(I.m, beta_eff, l, part, ws, vl, interp_stage_sum(st, part, mi));
+}
 __global__ void __launch_bounds__(kThreads) kb_ibm_sample_t(BatchTables t, int cpar) {
   const int gm = blockIdx.x * blockDim.x + threadIdx.x;
   if (gm >= t.M) return;
@@ -672,22 +754,29 @@ void launch_repulsion(const MarkerArrays& m, float delta, float gamma, hipStream
   k_repulsion<<<blocks_for(m.M, kWavesPerBlock), kThreads, 0, s>>>(m, delta, gamma);
   GLSM_LAUNCH_CHECK();
 }
-// Interpolation form: 0 = by marker count, 1 = wavefront per marker,
-// 2 = thread per marker.  Both forms give the same bits.
+// Interpolation form: 0 = default, 1 = wavefront per marker, 2 = thread
+// per marker, 3 = staged through LDS.  All forms give the same bits.  The
+// default interpolates in the staged form (measured fastest from a few
+// hundred to ~30k markers); concentration sampling, which has the wavefront
+// and thread forms only, takes the thread form from kThreadFormMarkers
+// markers on (and in modes 2 and 3).
 namespace {
 int g_interp_mode = 0;
 constexpr int kThreadFormMarkers = 2048;  // from this many markers on, one thread per marker
-bool thread_form(int M) { return g_interp_mode == 2 || (g_interp_mode == 0 && M >= kThreadFormMarkers); }
+bool thread_form(int M) { return g_interp_mode >= 2 || (g_interp_mode == 0 && M >= kThreadFormMarkers); }
+bool staged_form() { return g_interp_mode == 0 || g_interp_mode == 3; }
 }  // namespace
 void set_interp_mode(int mode) {
-  if (mode < 0 || mode > 2) throw std::invalid_argument("interpolation mode must be 0, 1 or 2");
+  if (mode < 0 || mode > 3) throw std::invalid_argument("interpolation mode must be 0, 1, 2 or 3");
   g_interp_mode = mode;
 }
 
 void launch_ibm_interp(const MarkerArrays& m, const SpreadLists& sl, const float3* u, float beta_eff,
                        hipStream_t s) {
   if (m.M == 0) return;
-  if (thread_form(m.M))
+  if (staged_form())
+    k_ibm_interp_s<<<blocks_for(m.M, kStageMarkers), kThreads, 0, s>>>(m, sl, u, beta_eff);
+  else if (thread_form(m.M))
     k_ibm_interp_t<<<blocks_for(3L * m.M, kThreads), kThreads, 0, s>>>(m, sl, u, beta_eff);
   else
     k_ibm_interp<<<blocks_for(m.M, kWavesPerBlock), kThreads, 0, s>>>(m, sl, u, beta_eff);
@@ -737,7 +826,9 @@ void launch_repulsion_batch(const BatchTables& t, hipStream_t s) {
 }
 void launch_ibm_interp_batch(const BatchTables& t, hipStream_t s) {
   if (t.M == 0) return;
-  if (thread_form(t.M))
+  if (staged_form())
+    kb_ibm_interp_s<<<blocks_for(t.M, kStageMarkers), kThreads, 0, s>>>(t);
+  else if (thread_form(t.M))
     kb_ibm_interp_t<<<blocks_for(3L * t.M, kThreads), kThreads, 0, s>>>(t);
   else
     kb_ibm_interp<<<blocks_for(t.M, kWavesPerBlock), kThreads, 0, s>>>(t);

@@ -16,8 +16,9 @@
 //   3. relaxes to the second-order equilibrium with the Guo forcing term
 //      (fluid_kernels.cu:98-142) and stores the 19 post-collision values.
 // Algorithmic traffic per cell update: 19 loads + 19 stores of the storage
-// type (152 B in FP32, 76 B in FP16/BF16); u (12 B) is stored only in flagged
-// blocks and on the last sub-step.  Adjacent lanes own adjacent cells, so
+// type (152 B in FP32, 76 B in FP16/BF16); u (12 B) is stored only in the
+// cells some marker stencil covers (those the interpolation reads) and, for
+// every cell, on the last sub-step.  Adjacent lanes own adjacent cells, so
 // every plane access of a 64-lane wavefront is one contiguous run (shifted
 // by the stream offset).
 //
@@ -332,7 +333,7 @@ __device__ __forceinline__ void lbm_block(const LatticeGeom& g, const LbmParams&
     constexpr int q = decltype(qc)::value;
     v[q] = Store<S>::ld(elem_at<B32>(gin, a.src<q>(i, plane)));
   });
-  const bool store_u = write_u_all || flagged;
+  bool store_u = write_u_all;
   GLSM_GLOBAL Vec3* gu = gmem(reinterpret_cast<Vec3*>(u));
   if (flagged || const_force) {
     float Fx = 0.f, Fy = 0.f, Fz = 0.f;
@@ -344,6 +345,7 @@ __device__ __forceinline__ void lbm_block(const LatticeGeom& g, const LbmParams&
 #pragma clang fp contract(off)
       const GLSM_GLOBAL int* rp = gmem(reinterpret_cast<const int*>(sl.rng)) + 2 * i;
       const int e0 = rp[0], e1 = e0 + rp[1];
+      store_u = store_u || e1 > e0;  // a stencil cell: the interpolation reads its u
       static_assert(sizeof(SpreadEntry) == 8 && offsetof(SpreadEntry, w) == 4, "entries are (int, float bits)");
       const GLSM_GLOBAL int* ent = gmem(reinterpret_cast<const int*>(sl.ent));
       const GLSM_GLOBAL float* gF = gmem(reinterpret_cast<const float*>(Fl));

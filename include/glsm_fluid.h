@@ -188,9 +188,13 @@ int glsm_fluid_write(glsm_fluid* f, int field, const void* host, size_t bytes);
 int glsm_copy_bandwidth(size_t bytes, int reps, double* gbps);
 
 /* Extension (measurement and tests): form of the IBM interpolation and
-   concentration sampling kernels, 0 = by marker count (default), 1 = one
-   wavefront per marker, 2 = one thread per marker.  Every form sums the 64
-   stencil products in the same order, so results are identical. */
+   concentration sampling kernels, 0 = default (interpolation staged as in
+   3; sampling by marker count: one wavefront per marker below 2048 markers,
+   else one thread per marker), 1 = one wavefront per marker, 2 = one
+   thread per marker, 3 = products staged in LDS by four threads per marker
+   and summed by one per component (sampling in the thread form).  Every
+   form sums the 64 stencil products in the same order, so results are
+   identical. */
 int glsm_set_interp_mode(int mode);
 
 /* ---- Batch (include/batch.h; no reference counterpart): many independent

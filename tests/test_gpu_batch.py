@@ -207,14 +207,16 @@ def test_batch_rejects_bad_members(glsm):
         close(inst)
 
 
-def test_interpolation_forms_identical(glsm):
-    """One wavefront per marker (solo runs) and one thread per marker (batch)
-    sum the same 64 products in the same order: identical bits."""
+@pytest.mark.parametrize("mode,name", [(2, "thread form"), (3, "staged form")])
+def test_interpolation_forms_identical(glsm, mode, name):
+    """One wavefront per marker (solo runs), one thread per marker and the
+    LDS-staged form (batch and solo) sum the same 64 products in the same
+    order: identical bits."""
     G = glsm
     try:
         G.set_interp_mode(1)
         ref = [solo(G, s, 0, 8) for s in SPECS]
-        G.set_interp_mode(2)
+        G.set_interp_mode(mode)
         b = G.Batch("f32")
         insts = [build(G, s) for s in SPECS]
         try:
@@ -222,11 +224,11 @@ def test_interpolation_forms_identical(glsm):
                 b.add(*inst)
             b.step(8)
             for k, inst in enumerate(insts):
-                assert_same(state(inst), ref[k], f"thread form, member {k}")
+                assert_same(state(inst), ref[k], f"{name}, member {k}")
         finally:
             b.close()
             for inst in insts:
                 close(inst)
-        assert_same(solo(G, SPECS[1], 0, 8), ref[1], "thread form, solo")
+        assert_same(solo(G, SPECS[1], 0, 8), ref[1], f"{name}, solo")
     finally:
         G.set_interp_mode(0)
