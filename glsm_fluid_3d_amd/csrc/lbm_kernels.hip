@@ -108,6 +108,11 @@ struct Store<F16_DEV> {
   }
   template <bool NT, class P>
   __device__ static float st(P p, float v) {
+    // v is rounded to FP32 first, then to FP16: the empty asm keeps the
+    // compiler from fusing the multiply-add that produced v with the
+    // conversion (a mixed-precision FMA rounds once, straight to FP16), a
+    // choice it would otherwise make differently in different kernels
+    asm("" : "+v"(v));
     const _Float16 h = (_Float16)v;
     put<T, NT>(p, h);
     return (float)h;
