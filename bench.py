@@ -113,7 +113,9 @@ def run_batch(G, storage, steps, warmup, count=64, tune=True):
            "gels": sum(len(g) for g, _, _ in insts), "markers": sum(c.M for _, c, _ in insts),
            "launch": shape, "hip_graph": True,
            "roofline": roofline(cells, 2 * 19 * G.STORAGE_BYTES[storage], lbm_ms)}
-    out["roofline"]["traffic"] = load_traffic(f"batch64_{storage}") if count == 64 else None
+    per_shape = load_traffic(f"batch64_{storage}") if count == 64 else None
+    out["roofline"]["traffic"] = (per_shape.get(f"nt{shape['nt']}_run{shape['run']}")
+                                  if isinstance(per_shape, dict) and shape.get("grid") == 0 else None)
     out["roofline"]["kernel"] = "kb_lbm_step (batched persistent pull stream + collide)"
     out["roofline"]["samples"] = samples
     b.close()
