@@ -3,11 +3,10 @@ exchanged through packed face buffers) reproduces the whole lattice's
 stream/collide sub-steps bit for bit, with walls and periodic in z, in FP32
 and FP16 storage, with uneven slab heights and a one-plane slab.
 
-References: in FP32, the whole lattice's own Fluid.sub_step with the marker
-force off (beta_eff = 0).  In FP16, one slab covering the whole lattice:
-the whole lattice's gel flags blocks for the forced collision, which with a
-zero force still rounds the last population update to FP32 before FP16,
-where the unforced collision of a slab rounds it to FP16 once.
+Reference: the whole lattice's own Fluid.sub_step with the marker force off
+(beta_eff = 0).  Its gel flags blocks for the forced collision, which with
+a zero force stores the same values as the slabs' unforced collision (in
+FP16 too: both round the FP32 result to FP16).
 """
 import numpy as np
 import pytest
@@ -40,21 +39,13 @@ def test_slab_chain_equals_whole_lattice(glsm, storage, per, split):
     parts = [(0, 5), (5, 5), (10, 4), (14, 5)] if split == "even" else [(0, 1), (1, 7), (8, 2), (10, 9)]
     assert sum(n for _, n in parts) == Nz
     slabs = [G.Slab(split_f, z0, nz) for z0, nz in parts]
-    if storage == "f16":
-        slabs.append(G.Slab(whole, 0, Nz))  # the reference chain of one
     try:
         for s in slabs:
             s.load(split_f)
-        if storage == "f16":
-            G.slab_chain_step(slabs[-1:], steps)
-            slabs[-1].store(whole)
-            chain = slabs[:-1]
-        else:
-            for _ in range(steps):
-                whole.sub_step(0.0)
-            chain = slabs
-        G.slab_chain_step(chain, steps)
-        for s in chain:
+        for _ in range(steps):
+            whole.sub_step(0.0)
+        G.slab_chain_step(slabs, steps)
+        for s in slabs:
             s.store(split_f)
         np.testing.assert_array_equal(split_f.read("fpost"), whole.read("fpost"))
     finally:
