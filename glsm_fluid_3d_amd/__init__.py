@@ -163,7 +163,7 @@ def init(device: int = 0):
 def set_interp_mode(mode: int):
     """0 = default (staged interpolation; sampling by marker count),
     1 = wavefront per marker, 2 = thread per marker, 3 = products staged in
-    LDS (four threads per marker)."""
+    LDS (eight threads per marker)."""
     _check(load().glsm_set_interp_mode(int(mode)), "glsm_set_interp_mode")
 
 

@@ -261,40 +261,48 @@ __device__ __forceinline__ void interp_component(const MarkerArrays& m, const Sp
 }
 
 // Staged form: a work-group takes kStageMarkers markers.  Phase 1, one
-// thread per (marker, 16-entry quarter of the stencil): the quarter's table
-// entries, then its 16 velocities, are loaded together and the 48 products
+// thread per (marker, kStageEntries-entry part of the stencil): the part's
+// table entries, then its velocities, are loaded together and the products
 // stored in LDS ([component][entry][marker], so both phases access
 // consecutive words across a wavefront).  Phase 2, one thread per (marker,
 // component): the 64 products added in stencil order from LDS.  Each marker
 // then waits for two dependent rounds of loads instead of eight, and every
 // table load is coalesced.  Same products, same order, same bits as the
 // other forms (cells outside the lattice add +-0 as in the thread form).
-constexpr int kStageMarkers = kThreads / 4;
+// 8 parts of 8 entries (32 markers, 24 KB of LDS per work-group) measured
+// fastest of 4, 8 and 16 parts from 436 to 29k markers.
+#ifndef GLSM_STAGE_PARTS
+#define GLSM_STAGE_PARTS 8
+#endif
+constexpr int kStageParts = GLSM_STAGE_PARTS;             // threads per marker in phase 1
+constexpr int kStageMarkers = kThreads / kStageParts;     // markers per work-group
+constexpr int kStageEntries = 64 / kStageParts;           // stencil entries per phase-1 thread
+static_assert(kStageParts >= 3 && 64 % kStageParts == 0, "phase 2 needs a thread per component");
 struct InterpStage {
   float p[3][64][kStageMarkers];
 };
 __device__ __forceinline__ void interp_stage_load(const SpreadLists& sl, int M, const float3* __restrict__ u, int l,
-                                                  int quarter, int mi, InterpStage& st) {
+                                                  int part, int mi, InterpStage& st) {
   const GLSM_GLOBAL int* ctab = gmem(sl.ctab);
   const GLSM_GLOBAL float* rtab = gmem(sl.rtab);
   const GLSM_GLOBAL float* gu = gmem(reinterpret_cast<const float*>(u));
-  const int k0 = 16 * quarter;
-  int cell[16];
-  float w[16], fx[16], fy[16], fz[16];
+  const int k0 = kStageEntries * part;
+  int cell[kStageEntries];
+  float w[kStageEntries], fx[kStageEntries], fy[kStageEntries], fz[kStageEntries];
 #pragma unroll
-  for (int j = 0; j < 16; ++j) {
+  for (int j = 0; j < kStageEntries; ++j) {
     const size_t t = (size_t)(k0 + j) * M + l;
     cell[j] = max(ctab[t], 0);
     w[j] = rtab[t];
   }
 #pragma unroll
-  for (int j = 0; j < 16; ++j) {
+  for (int j = 0; j < kStageEntries; ++j) {
     fx[j] = gu[3 * (size_t)cell[j]];
     fy[j] = gu[3 * (size_t)cell[j] + 1];
     fz[j] = gu[3 * (size_t)cell[j] + 2];
   }
 #pragma unroll
-  for (int j = 0; j < 16; ++j) {
+  for (int j = 0; j < kStageEntries; ++j) {
     st.p[0][k0 + j][mi] = fx[j] * w[j];
     st.p[1][k0 + j][mi] = fy[j] * w[j];
     st.p[2][k0 + j][mi] = fz[j] * w[j];

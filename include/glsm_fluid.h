@@ -191,7 +191,7 @@ int glsm_copy_bandwidth(size_t bytes, int reps, double* gbps);
    concentration sampling kernels, 0 = default (interpolation staged as in
    3; sampling by marker count: one wavefront per marker below 2048 markers,
    else one thread per marker), 1 = one wavefront per marker, 2 = one
-   thread per marker, 3 = products staged in LDS by four threads per marker
+   thread per marker, 3 = products staged in LDS by eight threads per marker
    and summed by one per component (sampling in the thread form).  Every
    form sums the 64 stencil products in the same order, so results are
    identical. */
