@@ -141,6 +141,28 @@ void Batch::setLaunch(const Launch& l) {
   dropGraphs();
 }
 
+// The parameter values a member's device descriptors copy from its objects.
+std::vector<char> Batch::memberParams(const Member& m) const {
+  std::vector<char> out;
+  auto put = [&](const void* p, size_t n) {
+    const char* c = static_cast<const char*>(p);
+    out.insert(out.end(), c, c + n);
+  };
+  const Fluid& f = *m.fluid;
+  const glsm::LbmParams p = glsm::fluid_lbm_params(f);
+  const glsm::ScalarParams sp{f.h_fp->dt, f.h_fp->D, f.h_fp->dx, f.h_fp->dy, f.h_fp->dz};
+  const float vals[6] = {m.coupler->h_cp->gel_to_lbm_vel, f.h_fp->beta, f.ramp_steps, m.coupler->h_cp->delta,
+                         m.coupler->h_cp->gamma, 0.f};
+  put(&p, sizeof p);
+  put(&sp, sizeof sp);
+  put(vals, sizeof vals);
+  for (Gel* g : *m.gels) {
+    const glsm::GelView v = glsm::gel_view(*g);
+    put(&v, sizeof v);
+  }
+  return out;
+}
+
 // Device tables from the members' objects; the members' current buffers
 // are bound to parity (pf_, pc_).
 void Batch::rebuild() {
@@ -161,6 +183,7 @@ void Batch::rebuild() {
     m.ftab[pf_ ^ 1] = f.d_fnext;
     m.ctab[pc_] = f.d_c1;
     m.ctab[pc_ ^ 1] = f.d_c2;
+    m.params = memberParams(m);
     glsm::BatchInst b{};
     b.g = glsm::fluid_geom(f);
     b.p = glsm::fluid_lbm_params(f);
@@ -277,7 +300,7 @@ void Batch::step(int nsteps) {
   if (nsteps < 0) throw std::invalid_argument("Batch::step: negative step count");
   if (members_.empty()) return;
   for (const Member& m : members_)
-    if (m.fluid->d_f != m.ftab[pf_] || m.fluid->d_c1 != m.ctab[pc_]) dirty_ = true;
+    if (m.fluid->d_f != m.ftab[pf_] || m.fluid->d_c1 != m.ctab[pc_] || memberParams(m) != m.params) dirty_ = true;
   if (dirty_) rebuild();
   hipStream_t s = glsm::stream();
   for (int n = 0; n < nsteps; ++n) {

@@ -20,6 +20,13 @@
 struct glsm_coupler {
   std::vector<Gel*> gels;  // Coupler keeps a reference to this vector
   Coupler* c = nullptr;
+  // the gels' device views, for glsm_system_step's one launch per gel phase;
+  // uploaded again whenever a gel's view changes (its parameters are set
+  // through its own methods between steps)
+  glsm::GelView* d_views = nullptr;
+  std::vector<glsm::GelView> h_views;
+  int max_L = 0;
+  ~glsm_coupler() { glsm::dfree(d_views); }
 };
 
 namespace {
@@ -915,12 +922,29 @@ int glsm_system_step(glsm_gel* const* gels, int n, glsm_coupler* c, glsm_fluid* 
     Fluid* fl = F(f);
     REQUIRE((int)cp->gels.size() == n, "gel count differs from the coupler's");
     for (int i = 0; i < n; ++i) REQUIRE(G(gels[i]) == cp->gels[i], "gel order differs from the coupler's");
-    for (int i = 0; i < n; ++i) {
-      Gel* g = cp->gels[i];
-      g->stepElasticity(iter);
-      g->stepChemistry(iter);
-      g->recordCenter(iter);
+    // the gels' stepElasticity + stepChemistry (gel.cu:682-704), one launch
+    // per phase for all of them: the gels are independent, so this is the
+    // per-gel sequence reordered
+    if (n > 0) {
+      std::vector<glsm::GelView> views;
+      for (Gel* g : cp->gels) views.push_back(glsm::gel_view(*g));
+      if (!cp->d_views) {
+        cp->d_views = glsm::dalloc<glsm::GelView>(views.size());
+        for (Gel* g : cp->gels)
+          cp->max_L = std::max(cp->max_L, std::max(g->m_gelNodeGrid.x, std::max(g->m_gelNodeGrid.y, g->m_gelNodeGrid.z)));
+      }
+      if (views.size() != cp->h_views.size() ||
+          std::memcmp(views.data(), cp->h_views.data(), views.size() * sizeof(glsm::GelView)) != 0) {
+        // blocking copy: the stream (blocking with respect to the null
+        // stream) finishes the steps that read the old views first
+        GLSM_HIP(hipMemcpy(cp->d_views, views.data(), views.size() * sizeof(glsm::GelView), hipMemcpyHostToDevice));
+        cp->h_views = views;
+      }
+      glsm::launch_gel_step_multi(cp->d_views, n, cp->max_L, iter, glsm::stream());
+      if (iter % 5 == 0)
+        for (Gel* g : cp->gels) g->markPositionsChanged();  // as stepElasticity does
     }
+    for (int i = 0; i < n; ++i) cp->gels[i]->recordCenter(iter);
     cp->c->packFromGels();
     fl->stepVelocity(iter);
     fl->stepConcentration(iter);

@@ -35,6 +35,7 @@ double splitmix64_unit(uint64_t& s) { return (double)(splitmix64_next(s) >> 11) 
 
 glsm::GelView make_view(Gel& g) {
   glsm::GelView v;
+  std::memset(&v, 0, sizeof v);  // padding too: views are compared bytewise
   const GelParams& p = *g.m_hgp;
   v.LX = p.LX; v.LY = p.LY; v.LZ = p.LZ;
   v.gelType = p.gelType;

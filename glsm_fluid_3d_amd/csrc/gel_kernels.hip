@@ -540,4 +540,22 @@ void launch_gel_step_batch(const GelView* views, const int* gel_inst, const int*
   GLSM_LAUNCH_CHECK();
 }
 
+// the cadence decided on the host: the kernels read no iteration counters
+void launch_gel_step_multi(const GelView* views, int ngel, int max_node_grid, int iter, hipStream_t s) {
+  const int L = max_node_grid;
+  const int nodes_all = (L + 2) * (L + 2) * (L + 2), elems_all = (L + 1) * (L + 1) * (L + 1);
+  const int nodes_real = L * L * L, elems_int = (L - 1) * (L - 1) * (L - 1);
+  if (iter % 5 == 0) {
+    launch_op_batch<OP_SERVICE>(views, nullptr, nullptr, ngel, nodes_all, 1, s);
+    launch_op_batch<OP_PROPS>(views, nullptr, nullptr, ngel, elems_all, 1, s);
+    launch_op_batch<OP_CHEMB>(views, nullptr, nullptr, ngel, elems_all, 1, s);
+    launch_op_batch<OP_UNNORM>(views, nullptr, nullptr, ngel, nodes_real, 1, s);
+    launch_op_batch<OP_POS>(views, nullptr, nullptr, ngel, nodes_real, 1, s);
+    launch_op_batch<OP_TERMS>(views, nullptr, nullptr, ngel, elems_int, 1, s);
+  }
+  launch_op_batch<OP_SETZERO>(views, nullptr, nullptr, ngel, nodes_real, 1, s);
+  launch_op_batch<OP_CHEM>(views, nullptr, nullptr, ngel, elems_int, 1, s);
+  GLSM_LAUNCH_CHECK();
+}
+
 }  // namespace glsm

@@ -109,6 +109,9 @@ void launch_gel_record_center(const GelView& g, int slot, hipStream_t s);
 // memory, iters[gel_inst[g]] = the iteration of gel g's instance
 void launch_gel_step_batch(const GelView* views, const int* gel_inst, const int* iters, int ngel,
                            int max_node_grid, hipStream_t s);
+// the same for `ngel` gels at one host-side iteration, without the centre
+// record (Gel::recordCenter, one gel at a time): views in device memory
+void launch_gel_step_multi(const GelView* views, int ngel, int max_node_grid, int iter, hipStream_t s);
 
 // ---- coupler ----------------------------------------------------------------
 }  // namespace glsm

@@ -93,7 +93,12 @@ class Batch {
     // the tables are rebuilt
     void* ftab[2];
     float* ctab[2];
+    // the member's parameters as the device tables hold them (fluid, IBM,
+    // coupler and gel values); a change made through the objects' own
+    // setters between steps is seen and the tables are rebuilt
+    std::vector<char> params;
   };
+  std::vector<char> memberParams(const Member& m) const;
   void rebuild();
   bool listsNeeded() const;
   void enqueueStep(bool timed, bool lists);

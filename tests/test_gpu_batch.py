@@ -232,3 +232,65 @@ def test_interpolation_forms_identical(glsm, mode, name):
         assert_same(solo(G, SPECS[1], 0, 8), ref[1], f"{name}, solo")
     finally:
         G.set_interp_mode(0)
+
+
+def _per_object_step(gels, c, f, it):
+    """sim.cpp's loop body through the objects' own methods, one gel at a time."""
+    for g in gels:
+        g.step_elasticity(it)
+        g.step_chemistry(it)
+        g.record_center(it)
+    c.pack_from_gels()
+    f.step_velocity(it)
+    f.step_concentration(it)
+    c.apply_gel_repulsion()
+    c.scatter_to_gels()
+
+
+def _change_params(inst, k):
+    gels, c, f = inst
+    if k == 4:
+        gels[-1].set_chemistry_rates(0.9, 0.3)
+    if k == 7:
+        gels[0].set_anchor_z(2.5)
+        f.set_ibm(0.8, 10.0)
+
+
+def test_system_step_equals_per_object_steps_with_changes(glsm):
+    """glsm_system_step launches each gel phase once for all the coupler's
+    gels; it equals the objects' own per-gel calls bit for bit, also when
+    gel and IBM parameters change between steps."""
+    G = glsm
+    spec = SPECS[1]  # two gels
+    a, b = build(G, spec), build(G, spec)
+    try:
+        for it in range(11):
+            _change_params(a, it)
+            _change_params(b, it)
+            G.system_step(a[0], a[1], a[2], it)
+            _per_object_step(*b, it)
+        assert_same(state(a), state(b), "system step vs per-object steps")
+    finally:
+        close(a)
+        close(b)
+
+
+def test_batch_sees_parameter_changes(glsm):
+    """A member's gel and IBM parameters changed through its own objects
+    between batch steps are used from the next step on, as in its solo run."""
+    G = glsm
+    spec = SPECS[1]
+    solo_inst, member = build(G, spec), build(G, spec)
+    b = G.Batch("f32")
+    try:
+        b.add(*member)
+        for it in range(11):
+            _change_params(solo_inst, it)
+            _change_params(member, it)
+            G.system_step(solo_inst[0], solo_inst[1], solo_inst[2], it)
+            b.step(1)
+        assert_same(state(member), state(solo_inst), "batch member with parameter changes")
+    finally:
+        b.close()
+        close(solo_inst)
+        close(member)
