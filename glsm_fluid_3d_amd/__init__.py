@@ -86,6 +86,7 @@ SIGNATURES = {
     "glsm_fluid_read": (_i, [_vp, _i, _vp, _sz]),
     "glsm_fluid_write": (_i, [_vp, _i, _vp, _sz]),
     "glsm_set_interp_mode": (_i, [_i]),
+    "glsm_set_repulsion_cull": (_i, [_i]),
     "glsm_batch_create": (_i, [_i, _i, C.POINTER(_vp)]),
     "glsm_batch_destroy": (_i, [_vp]),
     "glsm_batch_add": (_i, [_vp, _vp, _vp, _i, _i, C.c_char_p]),
@@ -165,6 +166,12 @@ def set_interp_mode(mode: int):
     1 = wavefront per marker, 2 = thread per marker, 3 = products staged in
     LDS (eight threads per marker)."""
     _check(load().glsm_set_interp_mode(int(mode)), "glsm_set_interp_mode")
+
+
+def set_repulsion_cull(min_markers: int):
+    """Skip out-of-range gels in the repulsion from this many markers on
+    (default 4096; 0 = never); results are identical either way."""
+    _check(load().glsm_set_repulsion_cull(int(min_markers)), "glsm_set_repulsion_cull")
 
 
 def synchronize():

@@ -838,6 +838,13 @@ int glsm_set_interp_mode(int mode) {
   });
 }
 
+int glsm_set_repulsion_cull(int min_markers) {
+  return guarded([&] {
+    glsm::set_repulsion_cull(min_markers);
+    return GLSM_OK;
+  });
+}
+
 // ---------------------------------------------------------------- slabs
 struct glsm_slab {
   Slab* s = nullptr;

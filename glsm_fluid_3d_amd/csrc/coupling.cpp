@@ -1,6 +1,7 @@
 // coupling.cpp -- host side of the Coupler class (reference: src/coupling.cu).
 #include "coupling.h"
 
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -76,6 +77,8 @@ Coupler::Coupler(std::vector<Gel*>& g)
   d_bIndex_all_ = glsm::dalloc<int>(M);
   d_owner = glsm::dalloc<int>(M);
   d_desc_ = glsm::dalloc<GelMarkerDesc>(numGels);
+  d_gel_off_ = glsm::dalloc<int>(numGels + 1);
+  d_gel_box_ = glsm::dalloc<float>(6 * (size_t)std::max(numGels, 1));
   hipStream_t s = coupler_stream;
   GLSM_HIP(hipMemcpyAsync(d_cp, h_cp, sizeof(CouplerParams), hipMemcpyHostToDevice, s));
   if (M) {
@@ -84,6 +87,9 @@ Coupler::Coupler(std::vector<Gel*>& g)
   }
   if (numGels)
     GLSM_HIP(hipMemcpyAsync(d_desc_, desc.data(), numGels * sizeof(GelMarkerDesc), hipMemcpyHostToDevice, s));
+  std::vector<int> gel_off(h_offsets.begin(), h_offsets.end());
+  gel_off.push_back(sumGelBoundaryCount);
+  GLSM_HIP(hipMemcpyAsync(d_gel_off_, gel_off.data(), gel_off.size() * sizeof(int), hipMemcpyHostToDevice, s));
   GLSM_HIP(hipStreamSynchronize(s));
   m_seen_versions.assign(numGels, 0);
 }
@@ -92,7 +98,7 @@ Coupler::~Coupler() {
   (void)hipStreamSynchronize(coupler_stream);
   for (void* p : {(void*)d_cp, (void*)d_lag_all_, (void*)d_Ul_all_, (void*)d_Vl_all_, (void*)d_Fdrag_all_,
                   (void*)d_Frep_all_, (void*)d_Cl_all_, (void*)d_Sl_all_, (void*)d_Dl_all_, (void*)d_bIndex_all_,
-                  (void*)d_owner, (void*)d_desc_})
+                  (void*)d_owner, (void*)d_desc_, (void*)d_gel_off_, (void*)d_gel_box_})
     glsm::dfree(p);
   delete h_cp;
 }
@@ -116,7 +122,8 @@ void Coupler::scatterToGels() { glsm::launch_scatter(marker_arrays(*this, d_desc
 
 // coupling.cu:191-194
 void Coupler::applyGelRepulsion() {
-  glsm::launch_repulsion(marker_arrays(*this, d_desc_), h_cp->delta, h_cp->gamma, coupler_stream);
+  glsm::launch_repulsion(marker_arrays(*this, d_desc_), h_cp->delta, h_cp->gamma, coupler_stream, numGels,
+                         d_gel_off_, d_gel_box_);
 }
 
 namespace glsm {

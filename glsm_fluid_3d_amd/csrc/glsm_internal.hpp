@@ -152,7 +152,13 @@ MarkerArrays markers_with_gels(Coupler& c);  // with the gel table (pack / scatt
 
 void launch_pack(const MarkerArrays& m, float gel_to_lbm_vel, hipStream_t s);
 void launch_scatter(const MarkerArrays& m, hipStream_t s);
-void launch_repulsion(const MarkerArrays& m, float delta, float gamma, hipStream_t s);
+// With the gels' marker ranges (gel_off, ngel + 1 entries) and a box buffer
+// (6 floats per gel), large marker sets skip the gels out of range (same
+// bits as the full scan); set_repulsion_cull(n): from n markers on (0 =
+// never; a test and measurement switch)
+void launch_repulsion(const MarkerArrays& m, float delta, float gamma, hipStream_t s, int ngel = 0,
+                      const int* gel_off = nullptr, float* box = nullptr);
+void set_repulsion_cull(int min_markers);
 
 // ---- lattice ------------------------------------------------------------------
 struct LatticeGeom {

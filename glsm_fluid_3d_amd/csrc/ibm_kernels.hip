@@ -141,17 +141,17 @@ __device__ __forceinline__ void scatter_at(const MarkerArrays& m, int l) {
 // measured best of 1, 4, 8 and 16: 126.6, 72.0, 69.4 and 81.0 us at
 // 10,098 markers).
 constexpr int kRepAhead = 8;
-__device__ __forceinline__ void repulsion_wave(const MarkerArrays& m, float delta, float gamma, int a, int lane) {
+// partners [b0, b1) of marker a (position pa, owner oa), added to f in
+// increasing index
+__device__ __forceinline__ void repulsion_range(const MarkerArrays& m, float delta, float gamma, float3 pa, int oa,
+                                                int b0, int b1, int lane, float& fx, float& fy, float& fz) {
   const float rcut = 2.5f * gamma, rc2 = rcut * rcut, sig2 = gamma * gamma;
-  const float3 pa = m.lag[a];
-  const int oa = m.owner[a];
-  float fx = 0.f, fy = 0.f, fz = 0.f;
-  for (int base = 0; base < m.M; base += 64 * kRepAhead) {
+  for (int base = b0; base < b1; base += 64 * kRepAhead) {
     int ob[kRepAhead];
     float3 pb[kRepAhead];
 #pragma unroll
     for (int j = 0; j < kRepAhead; ++j) {
-      const int b = min(base + 64 * j + lane, m.M - 1);  // (a partner past the end is not counted)
+      const int b = min(base + 64 * j + lane, b1 - 1);  // (a partner past the end is not counted)
       ob[j] = m.owner[b];
       pb[j] = m.lag[b];
     }
@@ -160,7 +160,7 @@ __device__ __forceinline__ void repulsion_wave(const MarkerArrays& m, float delt
       const int b = base + 64 * j + lane;
       float cx = 0.f, cy = 0.f, cz = 0.f;
       bool hit = false;
-      if (b < m.M && ob[j] != oa) {
+      if (b < b1 && ob[j] != oa) {
         const float dx = pa.x - pb[j].x, dy = pa.y - pb[j].y, dz = pa.z - pb[j].z;
         const float d2 = dx * dx + dy * dy + dz * dz;
         if (d2 > 0.0f && d2 < rc2) {
@@ -185,7 +185,63 @@ __device__ __forceinline__ void repulsion_wave(const MarkerArrays& m, float delt
       }
     }
   }
+}
+__device__ __forceinline__ void repulsion_wave(const MarkerArrays& m, float delta, float gamma, int a, int lane) {
+  float fx = 0.f, fy = 0.f, fz = 0.f;
+  repulsion_range(m, delta, gamma, m.lag[a], m.owner[a], 0, m.M, lane, fx, fy, fz);
   if (lane == 0) m.Frep[a] = make_float3(0.f + fx, 0.f + fy, 0.f + fz);
+}
+// The same sum with the markers of each gel in turn (a gel's markers are
+// consecutive, so gel by gel is still increasing index), skipping a gel
+// whose markers' bounding box lies beyond the cut-off from marker a: none
+// of them would count (the 0.1% margin covers the rounding of both
+// distances).  gel_off[g] = first marker of gel g, gel_off[ngel] = M; box =
+// per gel (min x, y, z, max x, y, z).
+__device__ __forceinline__ void repulsion_wave_culled(const MarkerArrays& m, int ngel, const int* __restrict__ gel_off,
+                                                      const float* __restrict__ box, float delta, float gamma, int a,
+                                                      int lane) {
+  const float rcut = 2.5f * gamma, rc2 = rcut * rcut;
+  const float3 pa = m.lag[a];
+  const int oa = m.owner[a];
+  float fx = 0.f, fy = 0.f, fz = 0.f;
+  for (int g = 0; g < ngel; ++g) {
+    if (g == oa) continue;
+    const float* bx = box + 6 * g;
+    const float ex = fmaxf(fmaxf(bx[0] - pa.x, pa.x - bx[3]), 0.f);
+    const float ey = fmaxf(fmaxf(bx[1] - pa.y, pa.y - bx[4]), 0.f);
+    const float ez = fmaxf(fmaxf(bx[2] - pa.z, pa.z - bx[5]), 0.f);
+    if (ex * ex + ey * ey + ez * ez > rc2 * 1.001f) continue;
+    repulsion_range(m, delta, gamma, pa, oa, gel_off[g], gel_off[g + 1], lane, fx, fy, fz);
+  }
+  if (lane == 0) m.Frep[a] = make_float3(0.f + fx, 0.f + fy, 0.f + fz);
+}
+
+// per gel, the bounding box of its markers (one work-group per gel)
+__device__ __forceinline__ void gel_marker_box(const MarkerArrays& m, const int* __restrict__ gel_off,
+                                               float* __restrict__ box, int g) {
+  __shared__ float s[kWavesPerBlock][6];
+  float v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+  for (int b = gel_off[g] + threadIdx.x; b < gel_off[g + 1]; b += kThreads) {
+    const float3 p = m.lag[b];
+    v[0] = fminf(v[0], p.x); v[1] = fminf(v[1], p.y); v[2] = fminf(v[2], p.z);
+    v[3] = fmaxf(v[3], p.x); v[4] = fmaxf(v[4], p.y); v[5] = fmaxf(v[5], p.z);
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const float w = __shfl_xor(v[k], o, 64);
+      v[k] = k < 3 ? fminf(v[k], w) : fmaxf(v[k], w);
+    }
+  if ((threadIdx.x & 63) == 0)
+    for (int k = 0; k < 6; ++k) s[threadIdx.x >> 6][k] = v[k];
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const int k = threadIdx.x;
+    float r = s[0][k];
+    for (int w = 1; w < kWavesPerBlock; ++w) r = k < 3 ? fminf(r, s[w][k]) : fmaxf(r, s[w][k]);
+    box[6 * g + k] = r;
+  }
 }
 
 // ---------------------------------------------------------------- IBM
@@ -563,6 +619,14 @@ __global__ void __launch_bounds__(kThreads) k_scatter(MarkerArrays m) {
   const int l = blockIdx.x * blockDim.x + threadIdx.x;
   if (l < m.M) scatter_at(m, l);
 }
+__global__ void __launch_bounds__(kThreads) k_gel_marker_box(MarkerArrays m, const int* gel_off, float* box) {
+  gel_marker_box(m, gel_off, box, blockIdx.x);
+}
+__global__ void __launch_bounds__(kThreads) k_repulsion_culled(MarkerArrays m, int ngel, const int* gel_off,
+                                                               const float* box, float delta, float gamma) {
+  const int a = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  if (a < m.M) repulsion_wave_culled(m, ngel, gel_off, box, delta, gamma, a, threadIdx.x & 63);
+}
 __global__ void __launch_bounds__(kThreads) k_repulsion(MarkerArrays m, float delta, float gamma) {
   const int a = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
   if (a < m.M) repulsion_wave(m, delta, gamma, a, threadIdx.x & 63);  // whole wavefront
@@ -772,9 +836,22 @@ void launch_scatter(const MarkerArrays& m, hipStream_t s) {
   k_scatter<<<blocks_for(m.M, kThreads), kThreads, 0, s>>>(m);
   GLSM_LAUNCH_CHECK();
 }
-void launch_repulsion(const MarkerArrays& m, float delta, float gamma, hipStream_t s) {
+namespace {
+int g_rep_cull_markers = 4096;  // from this many markers on (and two gels), skip far gels
+}  // namespace
+void set_repulsion_cull(int min_markers) {
+  if (min_markers < 0) throw std::invalid_argument("repulsion cull threshold must be >= 0 (0 = never)");
+  g_rep_cull_markers = min_markers;
+}
+void launch_repulsion(const MarkerArrays& m, float delta, float gamma, hipStream_t s, int ngel, const int* gel_off,
+                      float* box) {
   if (m.M == 0) return;
-  k_repulsion<<<blocks_for(m.M, kWavesPerBlock), kThreads, 0, s>>>(m, delta, gamma);
+  if (gel_off && box && ngel > 1 && g_rep_cull_markers > 0 && m.M >= g_rep_cull_markers) {
+    k_gel_marker_box<<<ngel, kThreads, 0, s>>>(m, gel_off, box);
+    k_repulsion_culled<<<blocks_for(m.M, kWavesPerBlock), kThreads, 0, s>>>(m, ngel, gel_off, box, delta, gamma);
+  } else {
+    k_repulsion<<<blocks_for(m.M, kWavesPerBlock), kThreads, 0, s>>>(m, delta, gamma);
+  }
   GLSM_LAUNCH_CHECK();
 }
 // Interpolation form: 0 = default, 1 = wavefront per marker, 2 = thread

@@ -66,6 +66,8 @@ class Coupler {
 
  private:
   GelMarkerDesc* d_desc_ = nullptr;
+  int* d_gel_off_ = nullptr;    // first marker of each gel, then the total
+  float* d_gel_box_ = nullptr;  // per gel: its markers' bounding box (repulsion)
   std::vector<uint64_t> m_seen_versions;
   uint64_t m_lag_version = 0;
 };

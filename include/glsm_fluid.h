@@ -197,6 +197,13 @@ int glsm_copy_bandwidth(size_t bytes, int reps, double* gbps);
    identical. */
 int glsm_set_interp_mode(int mode);
 
+/* Extension (measurement and tests): from `min_markers` markers on (default
+   4096; 0 = never), Coupler::applyGelRepulsion skips, for each marker, the
+   gels whose markers' bounding box lies beyond the cut-off.  The sum is
+   still taken in increasing partner index, so results are identical to the
+   full scan. */
+int glsm_set_repulsion_cull(int min_markers);
+
 /* ---- Batch (include/batch.h; no reference counterpart): many independent
    instances, each a coupler's gels + the coupler + its fluid, advanced one
    time step at a time by one launch per phase over all of them (HIP-graph

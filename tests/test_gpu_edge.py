@@ -57,3 +57,36 @@ def test_batch_member_without_gels(glsm):
         member[2].close()
         member[1].close()
         close(other)
+
+
+def test_repulsion_skipping_far_gels_is_exact(glsm):
+    """Three gels, two within the repulsion cut-off of each other and one far
+    away: skipping the gels whose markers' bounding box is out of range gives
+    the full scan's forces bit for bit."""
+    G = glsm
+    specs = [((4, 4, 4), (5.0, 6.0, 6.0)), ((3, 4, 3), (9.6, 6.2, 5.9)), ((3, 3, 3), (20.0, 18.0, 17.0))]
+
+    def system():
+        gels = [G.Gel(s, p, gel_id=i + 1) for i, (s, p) in enumerate(specs)]
+        for i, g in enumerate(gels):
+            g.perturb_chemistry(100 + i, 0.01)
+        c = G.Coupler(gels)
+        return gels, c, G.Fluid((26, 22, 22), c)
+
+    out = []
+    for cull in (0, 1):
+        G.set_repulsion_cull(cull)
+        try:
+            gels, c, f = system()
+            for it in range(3):
+                G.system_step(gels, c, f, it)
+            out.append({"Frep": c.read("Frep"), "u": f.read("u"),
+                        **{f"g{k}.Fn_robin": g.read("Fn_robin") for k, g in enumerate(gels)}})
+            assert np.abs(out[-1]["Frep"]).max() > 0  # the two near gels repel
+            f.close()
+            c.close()
+            for g in gels:
+                g.close()
+        finally:
+            G.set_repulsion_cull(4096)
+    assert_same(out[1], out[0], "culled vs full repulsion")

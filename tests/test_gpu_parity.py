@@ -146,12 +146,17 @@ def two_gel_system(glsm, oracle, prec=32, per=0, box=(16, 12, 12), storage="f32"
     return gs, c, f, ogs, oc, of
 
 
-def test_pack_repulsion_scatter(glsm, oracle):
-    say("coupler pack/repulsion/scatter")
+@pytest.mark.parametrize("cull", [4096, 1])
+def test_pack_repulsion_scatter(glsm, oracle, cull):
+    say(f"coupler pack/repulsion/scatter (far gels skipped from {cull} markers)")
     gs, c, f, ogs, oc, of = two_gel_system(glsm, oracle)
-    for it in range(2):
-        glsm.system_step(gs, c, f, it)
-        oracle.system_step(ogs, oc, of, it)
+    glsm.set_repulsion_cull(cull)
+    try:
+        for it in range(2):
+            glsm.system_step(gs, c, f, it)
+            oracle.system_step(ogs, oc, of, it)
+    finally:
+        glsm.set_repulsion_cull(4096)
     for name in ("lag", "Vl", "Cl"):
         assert np.array_equal(c.read(name).astype(np.float64), oc.read(name)), name
     Fr, oFr = c.read("Frep").astype(np.float64), oc.read("Frep")
