@@ -20,6 +20,7 @@
 //     cell's contributions in marker order -- no atomics, the same bits every
 //     run.
 #include <algorithm>
+#include <atomic>
 #include <cstddef>
 
 #include "glsm_internal.hpp"
@@ -836,8 +837,10 @@ void launch_scatter(const MarkerArrays& m, hipStream_t s) {
   k_scatter<<<blocks_for(m.M, kThreads), kThreads, 0, s>>>(m);
   GLSM_LAUNCH_CHECK();
 }
+// Process-wide switches, read when a launch is issued: a setter called from
+// another host thread takes effect from the next launch on.
 namespace {
-int g_rep_cull_markers = 4096;  // from this many markers on (and two gels), skip far gels
+std::atomic<int> g_rep_cull_markers{4096};  // from this many markers on (and two gels), skip far gels
 }  // namespace
 void set_repulsion_cull(int min_markers) {
   if (min_markers < 0) throw std::invalid_argument("repulsion cull threshold must be >= 0 (0 = never)");
@@ -846,7 +849,8 @@ void set_repulsion_cull(int min_markers) {
 void launch_repulsion(const MarkerArrays& m, float delta, float gamma, hipStream_t s, int ngel, const int* gel_off,
                       float* box) {
   if (m.M == 0) return;
-  if (gel_off && box && ngel > 1 && g_rep_cull_markers > 0 && m.M >= g_rep_cull_markers) {
+  const int cull = g_rep_cull_markers.load(std::memory_order_relaxed);
+  if (gel_off && box && ngel > 1 && cull > 0 && m.M >= cull) {
     k_gel_marker_box<<<ngel, kThreads, 0, s>>>(m, gel_off, box);
     k_repulsion_culled<<<blocks_for(m.M, kWavesPerBlock), kThreads, 0, s>>>(m, ngel, gel_off, box, delta, gamma);
   } else {
@@ -861,10 +865,16 @@ void launch_repulsion(const MarkerArrays& m, float delta, float gamma, hipStream
 // and thread forms only, takes the thread form from kThreadFormMarkers
 // markers on (and in modes 2 and 3).
 namespace {
-int g_interp_mode = 0;
+std::atomic<int> g_interp_mode{0};
 constexpr int kThreadFormMarkers = 2048;  // from this many markers on, one thread per marker
-bool thread_form(int M) { return g_interp_mode >= 2 || (g_interp_mode == 0 && M >= kThreadFormMarkers); }
-bool staged_form() { return g_interp_mode == 0 || g_interp_mode == 3; }
+bool thread_form(int M) {
+  const int mode = g_interp_mode.load(std::memory_order_relaxed);
+  return mode >= 2 || (mode == 0 && M >= kThreadFormMarkers);
+}
+bool staged_form() {
+  const int mode = g_interp_mode.load(std::memory_order_relaxed);
+  return mode == 0 || mode == 3;
+}
 }  // namespace
 void set_interp_mode(int mode) {
   if (mode < 0 || mode > 3) throw std::invalid_argument("interpolation mode must be 0, 1, 2 or 3");

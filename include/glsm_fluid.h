@@ -194,14 +194,14 @@ int glsm_copy_bandwidth(size_t bytes, int reps, double* gbps);
    thread per marker, 3 = products staged in LDS by eight threads per marker
    and summed by one per component (sampling in the thread form).  Every
    form sums the 64 stencil products in the same order, so results are
-   identical. */
+   identical.  Process-wide; launches issued after the call use it. */
 int glsm_set_interp_mode(int mode);
 
 /* Extension (measurement and tests): from `min_markers` markers on (default
    4096; 0 = never), Coupler::applyGelRepulsion skips, for each marker, the
    gels whose markers' bounding box lies beyond the cut-off.  The sum is
    still taken in increasing partner index, so results are identical to the
-   full scan. */
+   full scan.  Process-wide; launches issued after the call use it. */
 int glsm_set_repulsion_cull(int min_markers);
 
 /* ---- Batch (include/batch.h; no reference counterpart): many independent
