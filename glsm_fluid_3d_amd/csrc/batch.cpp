@@ -379,20 +379,28 @@ Batch::Launch Batch::autotune(int reps, double* best_ms) {
   hipEvent_t e0, e1;
   GLSM_HIP(hipEventCreate(&e0));
   GLSM_HIP(hipEventCreate(&e1));
-  double best = 1e30;
-  Launch pick = launch_;
-  for (const Launch& c : cand) {
+  // one trial: a warm-up launch, then `reps` timed launches
+  auto trial = [&](const Launch& c) {
     const glsm::LbmLaunch sh = to_internal(c);
-    glsm::launch_lbm_step_batch(storage_, t, pf_, false, sh, s);  // warm-up
+    glsm::launch_lbm_step_batch(storage_, t, pf_, false, sh, s);
     GLSM_HIP(hipEventRecord(e0, s));
     for (int r = 0; r < reps; ++r) glsm::launch_lbm_step_batch(storage_, t, pf_, false, sh, s);
     GLSM_HIP(hipEventRecord(e1, s));
     GLSM_HIP(hipEventSynchronize(e1));
     float ms = 0;
     GLSM_HIP(hipEventElapsedTime(&ms, e0, e1));
-    const double per = ms / reps;
-    if (per < best) {
-      best = per;
+    return (double)ms / reps;
+  };
+  // Each candidate's time is the median of three trials, so that one
+  // mistimed trial, too fast or too slow, cannot decide the pick.
+  double best = 1e30;
+  Launch pick = launch_;
+  for (const Launch& c : cand) {
+    double tr[3];
+    for (double& x : tr) x = trial(c);
+    std::sort(tr, tr + 3);
+    if (tr[1] < best) {
+      best = tr[1];
       pick = c;
     }
   }

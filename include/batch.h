@@ -70,7 +70,8 @@ class Batch {
   Launch launch() const;
   void setLaunch(const Launch& l);
   // Times every candidate shape on the members' current state (state is
-  // not changed) and keeps the fastest; returns it and its mean launch time.
+  // not changed; each shape's time is the median of three trials of `reps`
+  // launches) and keeps the fastest; returns it and its mean launch time.
   Launch autotune(int reps, double* best_ms);
 
   // Mean duration of the batched stream/collide kernel (start/stop events
