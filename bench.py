@@ -40,16 +40,9 @@ def log(*a):
 
 def cube101_two_gels(mod, storage=None, prec=None, seed=1):
     """101^3 lattice, two 6^3 gels placed as in sim.cpp (x = 1/3, 2/3)."""
-    box = (50, 50, 50)
-    gels = [mod.Gel((6, 6, 6), (box[0] / 3.0, box[1] / 2.0, box[2] / 2.0), gel_id=1),
-            mod.Gel((6, 6, 6), (2.0 * box[0] / 3.0, box[1] / 2.0, box[2] / 2.0), gel_id=2)]
-    for i, g in enumerate(gels):
-        g.perturb_chemistry(seed + i, 0.01)
-    c = mod.Coupler(gels) if prec is None else mod.Coupler(gels, prec)
-    f = mod.Fluid(box, c)
-    if storage:
-        f.set_storage(storage)
-    return gels, c, f
+    from glsm_fluid_3d_amd import workloads as W
+
+    return W.two_gels_sim_cpp(mod, storage=storage, prec=prec, seed=seed)
 
 
 def gel_array_3x3(mod, storage=None):
@@ -66,27 +59,10 @@ def gel_array_3x3(mod, storage=None):
     return gels, c, f
 
 
-def four_gel_centres(seed=2, box=100.0, n=8, count=4):
-    """Seeded uniform centres of n^3-element gels, rejected when their boxes
-    come closer than one element (the 201^3 precision workload)."""
-    rng = np.random.default_rng(seed)
-    half = n / 2.0
-    out = []
-    while len(out) < count:
-        p = rng.uniform(half + 3, box - half - 3, 3)
-        if all(np.any(np.abs(p - q) >= n + 1) for q in out):
-            out.append(p)
-    return [tuple(float(v) for v in p) for p in out]
-
-
 def cube201_four_gels(mod, storage):
-    gels = [mod.Gel((8, 8, 8), p, gel_id=i + 1) for i, p in enumerate(four_gel_centres())]
-    for i, g in enumerate(gels):
-        g.perturb_chemistry(2 + i, 0.01)
-    c = mod.Coupler(gels)
-    f = mod.Fluid((100, 100, 100), c)
-    f.set_storage(storage)
-    return gels, c, f
+    from glsm_fluid_3d_amd import workloads as W
+
+    return W.four_gels_201(mod, storage=storage)
 
 
 def run_batch(G, storage, steps, warmup, count=64, tune=True):

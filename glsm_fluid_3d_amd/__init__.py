@@ -27,7 +27,7 @@ GEL_FIELDS = [
 GEL_FIELD = {n: i for i, n in enumerate(GEL_FIELDS)}
 CP_FIELDS = ["lag", "Ul", "Vl", "Fdrag", "Frep", "Cl", "Dl", "Sl", "owner", "bIndex"]
 CP_FIELD = {n: i for i, n in enumerate(CP_FIELDS)}
-FL_FIELDS = ["u", "c", "f", "fpost"]
+FL_FIELDS = ["u", "c", "f", "fpost", "rho"]
 FL_FIELD = {n: i for i, n in enumerate(FL_FIELDS)}
 STORAGE = {"f32full": 0, "f32": 1, "f16": 2, "bf16": 3}
 STORAGE_BYTES = {"f32full": 4, "f32": 4, "f16": 2, "bf16": 2}
@@ -56,6 +56,7 @@ SIGNATURES = {
     "glsm_gel_dims": (_i, [_vp, _pi, _pi, _pi, _pi]),
     "glsm_gel_perturb_chemistry": (_i, [_vp, C.c_uint64, _d]),
     "glsm_gel_set_chemistry_rates": (_i, [_vp, _d, _d]),
+    "glsm_gel_check_finite": (_i, [_vp, C.POINTER(C.c_longlong)]),
     "glsm_gel_field_bytes": (_sz, [_vp, _i]),
     "glsm_gel_read": (_i, [_vp, _i, _vp, _sz]),
     "glsm_gel_write": (_i, [_vp, _i, _vp, _sz]),
@@ -76,6 +77,7 @@ SIGNATURES = {
     "glsm_fluid_save_state": (_i, [_vp, C.c_char_p]),
     "glsm_fluid_conservation": (_i, [_vp, C.POINTER(_d)]),
     "glsm_fluid_load_state": (_i, [_vp, C.c_char_p]),
+    "glsm_fluid_check_finite": (_i, [_vp, C.POINTER(C.c_longlong), C.POINTER(_d), C.POINTER(_d)]),
     "glsm_fluid_set_periodic": (_i, [_vp, _i, _i, _i]),
     "glsm_fluid_set_ibm": (_i, [_vp, _f, _f]),
     "glsm_fluid_set_storage": (_i, [_vp, _i, _i]),
@@ -266,6 +268,12 @@ class Gel:
         """Rate constants f, ep; steady state solved again, chemistry reset to it."""
         _check(load().glsm_gel_set_chemistry_rates(self.h, float(f), float(ep)), "set_chemistry_rates")
 
+    def check_finite(self):
+        """Number of non-finite values in rn, Veln, um, vm, wm."""
+        n = C.c_longlong()
+        _check(load().glsm_gel_check_finite(self.h, C.byref(n)), "gel check_finite")
+        return n.value
+
     def get_index(self, xi, yi, zi, size):
         return _check(load().glsm_gel_get_index(self.h, xi, yi, zi, size), "get_index")
 
@@ -357,6 +365,15 @@ class Fluid:
         _check(load().glsm_fluid_conservation(self.h, out), "conservation")
         return np.array(out[:], dtype=np.float64)
 
+    def check_finite(self):
+        """{"bad_cells": cells with a non-finite stored population, velocity
+        or concentration (an overflowed FP16 deviation is stored as Inf),
+        "max_abs_dev": largest finite |f_q - w_q|, "limit": largest
+        magnitude the storage represents}."""
+        n, m, lim = C.c_longlong(), C.c_double(), C.c_double()
+        _check(load().glsm_fluid_check_finite(self.h, C.byref(n), C.byref(m), C.byref(lim)), "fluid check_finite")
+        return {"bad_cells": n.value, "max_abs_dev": m.value, "limit": lim.value}
+
     def save_state(self, path):
         """Lossless binary snapshot (stored populations, u, c, params)."""
         _check(load().glsm_fluid_save_state(self.h, os.fsencode(path)), "save_state")
@@ -385,8 +402,10 @@ class Fluid:
         return ms.value, n.value
 
     def read(self, name):
-        shape = {"u": (self.N, 3), "c": (self.N,), "f": (19, self.N), "fpost": (19, self.N)}[name]
-        a = np.empty(shape, dtype=np.float32)
+        """u [N, 3], c [N], f / fpost [19, N] (float32); rho [N] (float64,
+        rho - 1 of the f populations summed in FP64 from the stored values)."""
+        shape = {"u": (self.N, 3), "c": (self.N,), "f": (19, self.N), "fpost": (19, self.N), "rho": (self.N,)}[name]
+        a = np.empty(shape, dtype=np.float64 if name == "rho" else np.float32)
         _check(load().glsm_fluid_read(self.h, FL_FIELD[name], a.ctypes.data, a.nbytes), f"fluid read {name}")
         return a
 

@@ -150,6 +150,7 @@ size_t fluid_field_bytes(Fluid* f, int field) {
     case GLSM_FL_C: return f->N * sizeof(float);
     case GLSM_FL_F:
     case GLSM_FL_FPOST: return 19ull * f->N * sizeof(float);
+    case GLSM_FL_RHO: return (size_t)f->N * sizeof(double);
     default: throw std::invalid_argument("unknown fluid field " + std::to_string(field));
   }
 }
@@ -450,6 +451,13 @@ int glsm_gel_read(glsm_gel* g, int field, void* host, size_t bytes) {
     return GLSM_OK;
   });
 }
+int glsm_gel_check_finite(glsm_gel* g, long long* bad_values) {
+  return guarded([&] {
+    REQUIRE(bad_values, "null output pointer");
+    *bad_values = G(g)->checkFinite();
+    return GLSM_OK;
+  });
+}
 int glsm_gel_write(glsm_gel* g, int field, const void* host, size_t bytes) {
   return guarded([&] {
     Gel* p = G(g);
@@ -460,7 +468,12 @@ int glsm_gel_write(glsm_gel* g, int field, const void* host, size_t bytes) {
     if (field == GLSM_GEL_UM) std::memcpy(p->m_hum, host, bytes);
     if (field == GLSM_GEL_VM) std::memcpy(p->m_hvm, host, bytes);
     if (field == GLSM_GEL_WM) std::memcpy(p->m_hwm, host, bytes);
-    if (field == GLSM_GEL_RN) p->copyDataToHost();
+    if (field == GLSM_GEL_RN) {
+      p->copyDataToHost();
+      // new marker positions: the IBM stencil tables and spreading lists of
+      // every coupler, fluid and batch holding this gel are rebuilt
+      p->markPositionsChanged();
+    }
     return GLSM_OK;
   });
 }
@@ -564,6 +577,13 @@ int glsm_fluid_conservation(glsm_fluid* f, double out[4]) {
     return GLSM_OK;
   });
 }
+int glsm_fluid_check_finite(glsm_fluid* f, long long* bad_cells, double* max_abs_dev, double* limit) {
+  return guarded([&] {
+    REQUIRE(bad_cells, "null output pointer");
+    *bad_cells = F(f)->checkFinite(max_abs_dev, limit);
+    return GLSM_OK;
+  });
+}
 int glsm_fluid_set_periodic(glsm_fluid* f, int px, int py, int pz) {
   return guarded([&] { F(f)->setPeriodic(px, py, pz); return GLSM_OK; });
 }
@@ -608,6 +628,7 @@ int glsm_fluid_read(glsm_fluid* f, int field, void* host, size_t bytes) {
       case GLSM_FL_C: d2h(host, p->d_c1, bytes); break;
       case GLSM_FL_F: p->readPopulations(static_cast<float*>(host)); break;
       case GLSM_FL_FPOST: p->readPostCollision(static_cast<float*>(host)); break;
+      case GLSM_FL_RHO: p->readDensity(static_cast<double*>(host)); break;
     }
     return GLSM_OK;
   });

@@ -27,7 +27,7 @@ glsm::MarkerArrays marker_arrays(Coupler& c, const GelMarkerDesc* desc) {
 }
 }  // namespace
 
-// coupling.cu:121-151 + setInitValue/copyDataToDevice (coupling.cu:76-120)
+// coupling.cu:50-80 + setInitValue/copyDataToDevice (coupling.cu:28-49)
 Coupler::Coupler(std::vector<Gel*>& g)
     : h_cp(nullptr), d_cp(nullptr), d_bIndex_all_(nullptr), d_lag_all_(nullptr), d_Ul_all_(nullptr),
       d_Vl_all_(nullptr), d_Fdrag_all_(nullptr), d_Frep_all_(nullptr), d_Cl_all_(nullptr), d_Dl_all_(nullptr),
@@ -103,7 +103,7 @@ Coupler::~Coupler() {
   delete h_cp;
 }
 
-// coupling.cu:153-169: all gels' boundary nodes in one launch
+// coupling.cu:82-98: all gels' boundary nodes in one launch
 void Coupler::packFromGels() {
   bool moved = false;
   for (int i = 0; i < numGels; ++i) {
@@ -117,10 +117,10 @@ void Coupler::packFromGels() {
   glsm::launch_pack(marker_arrays(*this, d_desc_), h_cp->gel_to_lbm_vel, coupler_stream);
 }
 
-// coupling.cu:171-189
+// coupling.cu:100-118
 void Coupler::scatterToGels() { glsm::launch_scatter(marker_arrays(*this, d_desc_), coupler_stream); }
 
-// coupling.cu:191-194
+// coupling.cu:120-123
 void Coupler::applyGelRepulsion() {
   glsm::launch_repulsion(marker_arrays(*this, d_desc_), h_cp->delta, h_cp->gamma, coupler_stream, numGels,
                          d_gel_off_, d_gel_box_);

@@ -199,6 +199,11 @@ void Fluid::setGoonValue(int time) {
 }
 
 void Fluid::setPeriodic(int px, int py, int pz) {
+  // a 4-wide delta stencil wrapped on an axis of fewer than 4 nodes would
+  // put two lanes of one marker on the same cell
+  const int3 n = fluidNodeGrid;
+  if ((px && n.x < 4) || (py && n.y < 4) || (pz && n.z < 4))
+    throw std::invalid_argument("setPeriodic: a periodic axis needs at least 4 nodes");
   h_fp->perX = px ? 1 : 0;
   h_fp->perY = py ? 1 : 0;
   h_fp->perZ = pz ? 1 : 0;
@@ -355,6 +360,38 @@ void Fluid::conservation(double out[4]) {
                            fluid_stream);
   GLSM_HIP(hipMemcpyAsync(out, dout, 4 * sizeof(double), hipMemcpyDeviceToHost, fluid_stream));
   GLSM_HIP(hipStreamSynchronize(fluid_stream));
+}
+
+long long Fluid::checkFinite(double* max_abs_dev, double* limit) {
+  // d_red_ holds 4 * nblk_ + 4 >= 8 doubles: the first two serve as counters
+  auto* out2 = reinterpret_cast<unsigned long long*>(d_red_);
+  glsm::launch_lbm_health(storage_, geom(*this), lbm_params(*h_fp, storage_, scale_exp_), d_f, plane, d_u, d_c1, out2,
+                          fluid_stream);
+  unsigned long long h[2] = {0, 0};
+  GLSM_HIP(hipMemcpyAsync(h, out2, sizeof h, hipMemcpyDeviceToHost, fluid_stream));
+  GLSM_HIP(hipStreamSynchronize(fluid_stream));
+  if (max_abs_dev) {
+    double m;
+    std::memcpy(&m, &h[1], sizeof m);
+    *max_abs_dev = m;
+  }
+  if (limit) {
+    switch (storage_) {
+      case F16_DEV: *limit = 65504.0 * std::ldexp(1.0, -scale_exp_); break;
+      case BF16_DEV: *limit = 3.3895313892515355e38; break;  // largest finite bfloat16
+      default: *limit = 3.4028234663852886e38;               // largest finite float
+    }
+  }
+  return (long long)h[0];
+}
+
+void Fluid::readDensity(double* host) {
+  double* tmp = glsm::dalloc<double>(N);
+  glsm::launch_lbm_density(storage_, geom(*this), lbm_params(*h_fp, storage_, scale_exp_), d_f, plane, tmp,
+                           fluid_stream);
+  GLSM_HIP(hipMemcpyAsync(host, tmp, N * sizeof(double), hipMemcpyDeviceToHost, fluid_stream));
+  GLSM_HIP(hipStreamSynchronize(fluid_stream));
+  glsm::dfree(tmp);
 }
 
 void Fluid::readPopulations(float* host) {

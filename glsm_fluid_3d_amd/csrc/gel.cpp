@@ -219,8 +219,23 @@ void Gel::freeDeviceMemory() {
                   (void*)m_dFn_robin, (void*)m_dFdrag_robin, (void*)m_dVeln, (void*)m_dVels, (void*)m_drm_loc,
                   (void*)m_dnmSm, (void*)m_dVolm, (void*)m_dPrem, (void*)m_dmap_element, (void*)m_dmap_node,
                   (void*)m_dvm_center, (void*)m_dwm_center, (void*)m_drm_center, (void*)m_dFn_center,
-                  (void*)m_dVeln_center, (void*)m_dbIndex})
+                  (void*)m_dVeln_center, (void*)m_dbIndex, (void*)m_dcheck})
     glsm::dfree(p);
+}
+
+long long Gel::checkFinite() {
+  if (!m_dcheck) m_dcheck = glsm::dalloc<unsigned long long>(1);
+  const size_t nE = m_numGelElements, nN = m_numGelNodes;
+  glsm::launch_zero_u64(m_dcheck, 1, m_gel_stream);
+  glsm::launch_count_nonfinite(reinterpret_cast<const double*>(m_drn), 3 * nN, m_dcheck, m_gel_stream);
+  glsm::launch_count_nonfinite(reinterpret_cast<const double*>(m_dVeln), 3 * nN, m_dcheck, m_gel_stream);
+  glsm::launch_count_nonfinite(m_dum, nE, m_dcheck, m_gel_stream);
+  glsm::launch_count_nonfinite(m_dvm, nN, m_dcheck, m_gel_stream);
+  glsm::launch_count_nonfinite(m_dwm, nE, m_dcheck, m_gel_stream);
+  unsigned long long h = 0;
+  GLSM_HIP(hipMemcpyAsync(&h, m_dcheck, sizeof h, hipMemcpyDeviceToHost, m_gel_stream));
+  GLSM_HIP(hipStreamSynchronize(m_gel_stream));
+  return (long long)h;
 }
 
 int Gel::get_index(int xi, int yi, int zi, int size) {
@@ -444,7 +459,9 @@ void Gel::writeFiles(int iter) {
   }
 }
 
-// gel.cu:731-747
+// Extension: rate constants f and ep of GelParams (gel.cu:633-652), with the
+// steady state solved again (gel.cu:424-443) and written to the interior
+// elements as setInitValue does (gel.cu:179-219)
 void Gel::setChemistryRates(double f, double ep) {
   if (!(f > 0.0) || !(ep > 0.0)) throw std::invalid_argument("setChemistryRates: f and ep must be positive");
   m_hgp->f = f;
@@ -462,6 +479,7 @@ void Gel::setChemistryRates(double f, double ep) {
   uploadChemistry(true);
 }
 
+// gel.cu:731-747
 void Gel::resetToQuiescent() {
   const int LX = m_gelNodeGrid.x, LY = m_gelNodeGrid.y, LZ = m_gelNodeGrid.z;
   for (int zi = 1; zi <= LZ; zi++)

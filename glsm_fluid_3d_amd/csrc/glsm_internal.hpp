@@ -230,6 +230,17 @@ void launch_lbm_moments(int storage, const LatticeGeom& g, const LbmParams& p, c
                         double* partial, double* out4, hipStream_t s);
 void launch_lbm_import(int storage, const LatticeGeom& g, const LbmParams& p, const float* in, void* fpost,
                        size_t plane, size_t in_plane, hipStream_t s);
+// health scan of a lattice: out2[0] = cells with a non-finite stored value,
+// velocity or concentration; out2[1] = bits of the largest finite
+// |f_q - w_q| (a non-negative double).  out2 is cleared first.
+void launch_lbm_health(int storage, const LatticeGeom& g, const LbmParams& p, const void* fpost, size_t plane,
+                       const float3* u, const float* c, unsigned long long* out2, hipStream_t s);
+// p[0..n) = 0, and *out += number of non-finite values of a[0..n)
+void launch_zero_u64(unsigned long long* p, int n, hipStream_t s);
+void launch_count_nonfinite(const double* a, size_t n, unsigned long long* out, hipStream_t s);
+// rho - 1 per cell (FP64) of the pre-collision populations of the next sub-step
+void launch_lbm_density(int storage, const LatticeGeom& g, const LbmParams& p, const void* fpost, size_t plane,
+                        double* out, hipStream_t s);
 
 struct ScalarParams {
   float dt, D, dx, dy, dz;

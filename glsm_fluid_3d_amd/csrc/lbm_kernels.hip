@@ -536,6 +536,93 @@ __global__ void __launch_bounds__(kThreads) k_reduce_partials(const double* __re
   block_sum_store<4>(acc, out);
 }
 
+// ---- health scan and density field ----
+// k_lbm_health: per cell, whether any of its 19 stored values, its velocity
+// or its concentration is not finite (a binary16 deviation whose magnitude
+// passed 65504 / 2^s was stored as +-Inf: saturation shows up here), and the
+// largest finite |f_q - w_q| in value units.  Each wavefront combines its
+// lanes with shuffles and adds its count / raises the maximum with one
+// integer atomic: a count and a maximum do not depend on the order of the
+// atomics.  The maximum is kept as the bit pattern of a non-negative double,
+// whose integer order is its numeric order.
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+template <int S>
+__global__ void __launch_bounds__(kThreads) k_lbm_health(LatticeGeom g, LbmParams p,
+                                                         const typename Store<S>::T* __restrict__ fpost,
+                                                         unsigned plane, const float3* __restrict__ u,
+                                                         const float* __restrict__ c,
+                                                         unsigned long long* __restrict__ out2) {
+  const unsigned i = blockIdx.x * kThreads + threadIdx.x;
+  unsigned long long bad = 0;
+  double mx = 0.0;
+  if (i < (unsigned)g.N) {
+    bool ok = true;
+#pragma unroll
+    for (int q = 0; q < 19; ++q) {
+      const float v = to_value<S>(Store<S>::ld(fpost + (unsigned)q * plane + i), p);
+      const float d = Store<S>::dev ? v : v - wq(p, q);
+      if (__builtin_isfinite(d))
+        mx = fmax(mx, fabs((double)d));
+      else
+        ok = false;
+    }
+    const float3 ui = u[i];
+    ok = ok && __builtin_isfinite(ui.x) && __builtin_isfinite(ui.y) && __builtin_isfinite(ui.z) &&
+         __builtin_isfinite(c[i]);
+    bad = ok ? 0ull : 1ull;
+  }
+  bad = wave_sum_u64(bad);
+  mx = wave_max(mx);
+  if ((threadIdx.x & 63u) == 0) {
+    if (bad) atomicAdd(out2, bad);
+    atomicMax(out2 + 1, (unsigned long long)__double_as_longlong(mx));
+  }
+}
+
+// n values of a device array that are not finite, added to *out
+__global__ void __launch_bounds__(kThreads) k_count_nonfinite(const double* __restrict__ a, size_t n,
+                                                              unsigned long long* __restrict__ out) {
+  const size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
+  unsigned long long bad = (i < n && !__builtin_isfinite(a[i])) ? 1ull : 0ull;
+  bad = wave_sum_u64(bad);
+  if ((threadIdx.x & 63u) == 0 && bad) atomicAdd(out, bad);
+}
+
+__global__ void k_fill_u64(unsigned long long* __restrict__ p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+
+// rho - 1 of the pre-collision populations of the next sub-step, per cell,
+// summed in FP64 from the stored values (deviation storage: sum of the
+// deviations, since the weights sum to 1)
+template <int S>
+__global__ void __launch_bounds__(kThreads) k_lbm_density(LatticeGeom g, LbmParams p,
+                                                          const typename Store<S>::T* __restrict__ fpost,
+                                                          unsigned plane, double* __restrict__ out) {
+  const unsigned i = blockIdx.x * kThreads + threadIdx.x;
+  if (i >= (unsigned)g.N) return;
+  int x, y, z;
+  decode_cell(g, (int)i, x, y, z);
+  const PullAddr a(g, x, y, z);
+  double m = Store<S>::dev ? 0.0 : -1.0;
+  Unroll<19>::run([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    m += (double)to_value<S>(Store<S>::ld(fpost + a.src<q>(i, plane)), p);
+  });
+  out[i] = m;
+}
+
 // plain device copy (16 B per lane): the reference point for achievable HBM
 // bandwidth and for calibrating the FETCH_SIZE / WRITE_SIZE counters
 __global__ void __launch_bounds__(kThreads) k_copy16(const float4* __restrict__ a, float4* __restrict__ b, size_t n4) {
@@ -679,6 +766,35 @@ void launch_lbm_moments(int storage, const LatticeGeom& g, const LbmParams& p, c
                                      partial)));
   GLSM_LAUNCH_CHECK();
   k_reduce_partials<<<1, kThreads, 0, s>>>(partial, nb, out4);
+  GLSM_LAUNCH_CHECK();
+}
+
+void launch_lbm_health(int storage, const LatticeGeom& g, const LbmParams& p, const void* fpost, size_t plane,
+                       const float3* u, const float* c, unsigned long long* out2, hipStream_t s) {
+  k_fill_u64<<<1, 64, 0, s>>>(out2, 2);
+  GLSM_LAUNCH_CHECK();
+  GLSM_STORAGE_DISPATCH(storage, (k_lbm_health<S><<<blocks_for(g.N), kThreads, 0, s>>>(
+                                     g, p, static_cast<const typename Store<S>::T*>(fpost), (unsigned)plane, u, c,
+                                     out2)));
+  GLSM_LAUNCH_CHECK();
+}
+
+void launch_zero_u64(unsigned long long* p, int n, hipStream_t s) {
+  if (n <= 0) return;
+  k_fill_u64<<<(n + 63) / 64, 64, 0, s>>>(p, n);
+  GLSM_LAUNCH_CHECK();
+}
+
+void launch_count_nonfinite(const double* a, size_t n, unsigned long long* out, hipStream_t s) {
+  if (n == 0) return;
+  k_count_nonfinite<<<(unsigned)((n + kThreads - 1) / kThreads), kThreads, 0, s>>>(a, n, out);
+  GLSM_LAUNCH_CHECK();
+}
+
+void launch_lbm_density(int storage, const LatticeGeom& g, const LbmParams& p, const void* fpost, size_t plane,
+                        double* out, hipStream_t s) {
+  GLSM_STORAGE_DISPATCH(storage, (k_lbm_density<S><<<blocks_for(g.N), kThreads, 0, s>>>(
+                                     g, p, static_cast<const typename Store<S>::T*>(fpost), (unsigned)plane, out)));
   GLSM_LAUNCH_CHECK();
 }
 

@@ -20,7 +20,7 @@ void set_device(int device) {
 }
 
 // One stream for every Gel/Coupler/Fluid: the reference's per-object streams
-// (gel.cu:674, fluid.cu:243, coupling.cu:198) are unordered with respect to
+// (gel.cu:674, fluid.cu:243, coupling.cu:127) are unordered with respect to
 // each other, while its drivers assume program order.  Default priority,
 // blocking with respect to the null stream, so a driver's plain hipMemcpy
 // sees all queued work.

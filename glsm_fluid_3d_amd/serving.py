@@ -11,7 +11,11 @@ Jobs (lattice, gels, rate constants, horizon) arrive over time.  The server
   * streams each job's .dat files (the reference's layout and cadence)
     through an asynchronous OutputStream into its own directory;
   * retires a job the step its horizon is reached and records its arrival,
-    admission and completion times.
+    admission and completion times;
+  * checks the active jobs' state every `health_every` steps and at
+    completion (non-finite populations, velocity, concentration or gel
+    fields; an FP16 deviation past the storage's range is stored as Inf and
+    counts) and stops a failed job at once, listing it as failed.
 The loop runs on the host between chunks of steps; a chunk never passes the
 nearest horizon, so completions are exact to the step.
 """
@@ -103,7 +107,8 @@ class Server:
     before a finished job's objects are released."""
 
     def __init__(self, mod, storage="f16", hbm_budget=None, max_active_cells=16_000_000, max_active_jobs=256,
-                 out_root=None, keep_output=True, writers=4, max_chunk=50, autotune=True, on_complete=None):
+                 out_root=None, keep_output=True, writers=4, max_chunk=50, autotune=True, on_complete=None,
+                 health_every=100):
         self.G = mod
         self.storage = storage
         free, total = mod.device_memory()
@@ -115,6 +120,7 @@ class Server:
         self.max_chunk = max_chunk
         self.autotune = autotune
         self.on_complete = on_complete
+        self.health_every = health_every
         self.batch = mod.Batch(storage)
         self.output = mod.OutputStream(writers) if out_root else None
         if self.output:
@@ -122,6 +128,8 @@ class Server:
         self.queue = deque()
         self.active = {}
         self.done = []
+        self.rejected = []
+        self.failed = []
         self.used_bytes = 0
         self.active_cells = 0
         self.cell_updates = 0
@@ -130,7 +138,16 @@ class Server:
 
     # ---- admission ---------------------------------------------------------
     def submit(self, job: W.Job):
-        self.queue.append(JobRecord(job, job_footprint(job, self.storage, self.out_root is not None)))
+        """Queue a job; one whose footprint exceeds the whole HBM budget could
+        never be admitted and would block every job behind it, so it is
+        rejected at once (listed in stats()["rejected"])."""
+        rec = JobRecord(job, job_footprint(job, self.storage, self.out_root is not None))
+        if rec.footprint > self.hbm_budget:
+            rec.extra["reason"] = f"footprint {rec.footprint} B exceeds the HBM budget {self.hbm_budget} B"
+            self.rejected.append(rec)
+            return False
+        self.queue.append(rec)
+        return True
 
     def _fits(self, rec: JobRecord) -> bool:
         if len(self.active) >= self.max_active_jobs:
@@ -161,27 +178,51 @@ class Server:
         return n
 
     # ---- completion --------------------------------------------------------
-    def _retire(self, now: float):
-        finished = [r for r in self.active.values() if self.batch.iteration(r.member) >= r.job.horizon]
-        for rec in finished:
-            self.batch.remove(rec.member)
-            gels, c, f = rec.objects
-            if self.output:
-                for o in gels + [f]:
-                    self.output.release(o)
-            if self.on_complete:
-                self.on_complete(rec, gels, c, f)
-            for o in gels + [f, c]:
-                o.close()
-            rec.objects = None
-            rec.completed = now
-            rec.steps = rec.job.horizon
-            del self.active[rec.member]
-            self.used_bytes -= rec.footprint
-            self.active_cells -= rec.job.cells
+    def _health(self, rec):
+        """None if the job's state is finite, else a description of what is not."""
+        gels, c, f = rec.objects
+        h = f.check_finite()
+        bad_gel = sum(g.check_finite() for g in gels)
+        if h["limit"] > 0:
+            rec.extra["range_used"] = max(rec.extra.get("range_used", 0.0), h["max_abs_dev"] / h["limit"])
+        if h["bad_cells"] or bad_gel:
+            return (f"non-finite state at step {self.batch.iteration(rec.member)}: {h['bad_cells']} lattice cells, "
+                    f"{bad_gel} gel values")
+        return None
+
+    def _release(self, rec, now, failure=None):
+        self.batch.remove(rec.member)
+        gels, c, f = rec.objects
+        if self.output:
+            for o in gels + [f]:
+                self.output.release(o)
+        if self.on_complete and failure is None:
+            self.on_complete(rec, gels, c, f)
+        for o in gels + [f, c]:
+            o.close()
+        rec.objects = None
+        rec.completed = now
+        del self.active[rec.member]
+        self.used_bytes -= rec.footprint
+        self.active_cells -= rec.job.cells
+        if failure is None:
             self.done.append(rec)
-            if self.out_root and not self.keep_output:
-                shutil.rmtree(rec.out_dir, ignore_errors=True)
+        else:
+            rec.extra["failure"] = failure
+            self.failed.append(rec)
+        if self.out_root and not self.keep_output:
+            shutil.rmtree(rec.out_dir, ignore_errors=True)
+
+    def _retire(self, now: float, check_all: bool = False):
+        for rec in list(self.active.values()):
+            it = self.batch.iteration(rec.member)
+            finished = it >= rec.job.horizon
+            if not (finished or check_all):
+                continue
+            failure = self._health(rec)
+            if failure is not None or finished:
+                rec.steps = it
+                self._release(rec, now, failure)
 
     # ---- the loop ----------------------------------------------------------
     def run(self, jobs, duration=None, clock=None, idle_sleep=0.002, progress_every=None, log=None):
@@ -220,7 +261,8 @@ class Server:
             self.cell_updates += self.active_cells * NSUB * n
             if clock is not None:
                 clock.advance(n)
-            self._retire(now_fn())
+            check = self.health_every and (self.steps // self.health_every) != ((self.steps - n) // self.health_every)
+            self._retire(now_fn(), check_all=bool(check))
             if progress_every and now >= next_report:
                 next_report += progress_every
                 log(f"[serve {now:7.1f} s] active {len(self.active)} queued {len(self.queue)} "
@@ -237,6 +279,8 @@ class Server:
         pct = lambda a, q: float(np.percentile(a, q)) if a.size else None
         out = {
             "completed": len(self.done), "active": len(self.active), "queued": len(self.queue),
+            "rejected": [r.job.job_id for r in self.rejected],
+            "failed": {r.job.job_id: r.extra["failure"] for r in self.failed},
             "latency_p50_s": pct(lat, 50), "latency_p99_s": pct(lat, 99),
             "queue_wait_p50_s": pct(wait, 50), "queue_wait_p99_s": pct(wait, 99),
             "mlups": self.cell_updates / max(self.elapsed, 1e-9) / 1e6 if hasattr(self, "elapsed") else None,

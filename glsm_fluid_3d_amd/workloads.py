@@ -123,3 +123,49 @@ def build(mod, inst: Instance, storage=None):
     if storage:
         f.set_storage(storage)
     return gels, c, f
+
+
+# ---- the fixed single-lattice workloads -----------------------------------
+def two_gels_sim_cpp(mod, storage=None, prec=None, seed=1, box=50, n=6):
+    """The reference default scene: a cubic box of side `box` (101^3 nodes for
+    50) with walls, two n^3-element gels at x = 1/3 and 2/3 of the box,
+    centred in y and z (sim.cpp's fractional placement, sim.cpp:26-30),
+    chemistry perturbed by 0.01 from splitmix64(seed + gel index).  `prec`
+    selects the oracle's fluid arithmetic (ignored by the library)."""
+    gels = [mod.Gel((n, n, n), (box / 3.0, box / 2.0, box / 2.0), gel_id=1),
+            mod.Gel((n, n, n), (2.0 * box / 3.0, box / 2.0, box / 2.0), gel_id=2)]
+    for i, g in enumerate(gels):
+        g.perturb_chemistry(seed + i, 0.01)
+    c = mod.Coupler(gels) if prec is None else mod.Coupler(gels, prec)
+    f = mod.Fluid((box, box, box), c)
+    if storage:
+        f.set_storage(storage)
+    return gels, c, f
+
+
+def four_gel_centres(seed=2, box=100.0, n=8, count=4):
+    """Seeded uniform centres of n^3-element gels in a cubic box, at least 3
+    from the walls, rejected when their boxes come closer than one element
+    (the 201^3 precision workload)."""
+    rng = np.random.default_rng(seed)
+    half = n / 2.0
+    out = []
+    while len(out) < count:
+        p = rng.uniform(half + 3, box - half - 3, 3)
+        if all(np.any(np.abs(p - q) >= n + 1) for q in out):
+            out.append(p)
+    return [tuple(float(v) for v in p) for p in out]
+
+
+def four_gels_201(mod, storage=None, prec=None):
+    """201^3 nodes (box 100) with walls, four 8^3-element gels at
+    four_gel_centres(), chemistry perturbed by 0.01 from splitmix64(2 + gel
+    index)."""
+    gels = [mod.Gel((8, 8, 8), p, gel_id=i + 1) for i, p in enumerate(four_gel_centres())]
+    for i, g in enumerate(gels):
+        g.perturb_chemistry(2 + i, 0.01)
+    c = mod.Coupler(gels) if prec is None else mod.Coupler(gels, prec)
+    f = mod.Fluid((100, 100, 100), c)
+    if storage:
+        f.set_storage(storage)
+    return gels, c, f

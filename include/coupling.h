@@ -24,14 +24,14 @@ struct GelMarkerDesc;  // device table entry, defined in the library
 
 class Coupler {
  public:
-  explicit Coupler(std::vector<Gel*>& gels);  // coupling.cu:121-151
+  explicit Coupler(std::vector<Gel*>& gels);  // coupling.cu:50-80
   ~Coupler();
   Coupler(const Coupler&) = delete;
   Coupler& operator=(const Coupler&) = delete;
 
-  void packFromGels();       // coupling.cu:153-169
-  void scatterToGels();      // coupling.cu:171-189
-  void applyGelRepulsion();  // coupling.cu:191-194
+  void packFromGels();       // coupling.cu:82-98
+  void scatterToGels();      // coupling.cu:100-118
+  void applyGelRepulsion();  // coupling.cu:120-123
 
   // Bumped by packFromGels when the packed marker positions may differ from
   // the previous pack (some gel moved); the fluid re-bins its spreading

@@ -94,6 +94,18 @@ class Fluid {
   // Deterministic; waits for the fluid stream.
   void conservation(double out[4]);
 
+  // Extension (failure detection, the check the reference's drivers make
+  // on their own: sim_gelonly.cpp:112-115): number of cells whose stored
+  // populations, velocity or concentration hold a non-finite value (a
+  // binary16 deviation that overflowed is stored as Inf and counts), the
+  // largest finite |f_q - w_q| (max_abs_dev) and the largest magnitude the
+  // storage represents (limit: 65504 / 2^scale_exp for F16_DEV).  Waits for
+  // the fluid stream.
+  long long checkFinite(double* max_abs_dev = nullptr, double* limit = nullptr);
+  // rho - 1 per cell (N doubles) of the pre-collision populations of the
+  // next sub-step, summed in FP64 from the stored values.
+  void readDensity(double* hostN);
+
   // Pre-collision populations f_k(x,q) (what the reference's d_f holds) and
   // their inverse; host arrays of 19*N floats, plane-major.
   void readPopulations(float* host19N);

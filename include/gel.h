@@ -63,6 +63,12 @@ class Gel {
   // chemistry set to it, as at construction (gel.cu:179-219).
   void setChemistryRates(double f, double ep);
 
+  // Extension (failure detection, as the reference's drivers check vm and
+  // the tip position for NaN: sim_gelonly.cpp:102-115): number of
+  // non-finite values in rn, Veln (node arrays) and um, vm, wm.  Waits for
+  // the gel stream.
+  long long checkFinite();
+
   int get_index(int xi, int yi, int zi, int size);  // gel.cu:116-119
   int idx3(int x, int y, int z, int Nx, int Ny);    // gel.cu:156-159
   double fu_h(double u, double v, double w, double phi);            // gel.cu:445-450
@@ -161,4 +167,5 @@ class Gel {
   void setInitValue();
   void uploadChemistry(bool with_w);
   uint64_t m_rn_version = 1;
+  unsigned long long* m_dcheck = nullptr;  // checkFinite's counter
 };

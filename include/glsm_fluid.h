@@ -70,6 +70,8 @@ enum glsm_fluid_field {
                       next sub-step (reference d_f after the swap) */
   GLSM_FL_FPOST,   /* float[19][N]: post-collision populations (the stored
                       state of the pull scheme) */
+  GLSM_FL_RHO,     /* double: rho - 1 of the GLSM_FL_F populations, summed in
+                      FP64 from the stored values (read-only; extension) */
   GLSM_FL_NFIELDS
 };
 
@@ -115,17 +117,21 @@ int glsm_gel_perturb_chemistry(glsm_gel* g, uint64_t seed, double amplitude);
    state is solved again (gel.cu:424-443) and the interior chemistry set to
    it (gel.cu:179-219).  Call before stepping. */
 int glsm_gel_set_chemistry_rates(glsm_gel* g, double f, double ep);
+/* Extension (failure detection; the reference's drivers test vm and the
+   tip position for NaN themselves, sim_gelonly.cpp:102-115): number of
+   non-finite values in the gel's rn, Veln, um, vm and wm. */
+int glsm_gel_check_finite(glsm_gel* g, long long* bad_values);
 size_t glsm_gel_field_bytes(glsm_gel* g, int field);
 int glsm_gel_read(glsm_gel* g, int field, void* host, size_t bytes);
 int glsm_gel_write(glsm_gel* g, int field, const void* host, size_t bytes);
 
 /* ---- Coupler: src/coupling.h:14-58 ---- */
-/* Coupler::Coupler(std::vector<Gel*>&) coupling.cu:121-151 */
+/* Coupler::Coupler(std::vector<Gel*>&) coupling.cu:50-80 */
 int glsm_coupler_create(glsm_gel* const* gels, int num_gels, glsm_coupler** out);
-int glsm_coupler_destroy(glsm_coupler* c);                  /* coupling.cu:232-235 */
-int glsm_coupler_pack_from_gels(glsm_coupler* c);           /* coupling.cu:153-169 */
-int glsm_coupler_scatter_to_gels(glsm_coupler* c);          /* coupling.cu:171-189 */
-int glsm_coupler_apply_gel_repulsion(glsm_coupler* c);      /* coupling.cu:191-194 */
+int glsm_coupler_destroy(glsm_coupler* c);                  /* coupling.cu:161-164 */
+int glsm_coupler_pack_from_gels(glsm_coupler* c);           /* coupling.cu:82-98 */
+int glsm_coupler_scatter_to_gels(glsm_coupler* c);          /* coupling.cu:100-118 */
+int glsm_coupler_apply_gel_repulsion(glsm_coupler* c);      /* coupling.cu:120-123 */
 int glsm_coupler_marker_count(glsm_coupler* c);             /* sumGelBoundaryCount */
 size_t glsm_coupler_field_bytes(glsm_coupler* c, int field);
 int glsm_coupler_read(glsm_coupler* c, int field, void* host, size_t bytes);
@@ -147,6 +153,13 @@ int glsm_fluid_write_files(glsm_fluid* f, int iter);        /* fluid.cu:140-150 
    rho*u = sum_q f_q c_q as in k_macros (fluid_kernels.cu:98-111) without the
    F/2 term.  Per-cell sums in FP64, fixed-order block reduction. */
 int glsm_fluid_conservation(glsm_fluid* f, double out[4]);
+/* Extension (failure detection): bad_cells = cells whose stored populations,
+   velocity or concentration hold a non-finite value -- a binary16 deviation
+   beyond the storage's range is stored as Inf and counts; max_abs_dev (may
+   be NULL) = the largest finite |f_q - w_q| of the stored state; limit (may
+   be NULL) = the largest magnitude the storage represents (65504 / 2^s for
+   FP16 deviations scaled by 2^s).  One pass over the lattice. */
+int glsm_fluid_check_finite(glsm_fluid* f, long long* bad_cells, double* max_abs_dev, double* limit);
 /* Extension: lossless binary snapshot of the stored populations (in their
    storage representation), u, c and the params; loading switches the storage
    mode to the saved one.  It complements Fluid::setGoonValue

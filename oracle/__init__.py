@@ -296,6 +296,8 @@ class Fluid:
         return lib().oracle_fluid_beta_eff(self.h, int(it))
 
     def read(self, name):
+        if name == "rho":  # rho - 1 of the pre-collision populations, FP64
+            return self.read("f").sum(axis=0) - 1.0
         shape = {"u": (self.N, 3), "c": (self.N,), "f": (19, self.N), "fpost": (19, self.N)}[name]
         a = np.empty(shape, dtype=np.float64)
         rc = lib().oracle_fluid_read(self.h, FL_FIELD[name], _dptr(a), a.size)

@@ -821,7 +821,7 @@ template <class R>
 class Coupler {
  public:
   using R3 = V3<R>;
-  // Coupler::Coupler + setInitValue, coupling.cu:121-151, 110-120
+  // Coupler::Coupler + setInitValue, coupling.cu:50-80, 39-49
   explicit Coupler(std::vector<Gel*> g) : gels(std::move(g)) {
     numGels = (int)gels.size();
     M = 0;
@@ -838,7 +838,7 @@ class Coupler {
     Fdrag.assign(M, R3{0, 0, 0}); Frep.assign(M, R3{0, 0, 0});
     Cl.assign(M, 0); Dl.assign(M, 0); Sl.assign(M, 0);
   }
-  // packFromGels + k_gather_boundary, coupling.cu:153-169, coupling_kernels.cu:42-59
+  // packFromGels + k_gather_boundary, coupling.cu:82-98, coupling_kernels.cu:42-59
   void packFromGels() {
     for (int g = 0; g < numGels; ++g) {
       Gel* gl = gels[g];
@@ -853,7 +853,7 @@ class Coupler {
     }
   }
   // scatterToGels + k_add_reaction_to_gel / k_add_drag_to_gel,
-  // coupling.cu:171-189, coupling_kernels.cu:8-34
+  // coupling.cu:100-118, coupling_kernels.cu:8-34
   void scatterToGels() {
     for (int g = 0; g < numGels; ++g) {
       Gel* gl = gels[g];
@@ -875,7 +875,7 @@ class Coupler {
     }
   }
   // applyGelRepulsion + k_gel_repulsion/bodyBodyInteraction,
-  // coupling.cu:191-194, coupling_kernels.cu:61-146 (pair order b = 0..M-1)
+  // coupling.cu:120-123, coupling_kernels.cu:61-146 (pair order b = 0..M-1)
   void applyGelRepulsion() {
     const R rcut = (R)2.5 * gamma, rc2 = rcut * rcut, sig2 = gamma * gamma;
     for (int a = 0; a < M; ++a) {

@@ -172,3 +172,30 @@ def test_snapshot_rejects_other_shapes(glsm, tmp_path):
             g.load_state(str(tmp_path / "absent.snap"))
     finally:
         close_all(g, f, c, g4, f4, c4)
+
+
+def test_bodycenter_layout_byte_exact(glsm, tmp_path, monkeypatch):
+    """gel<id>bodycenter.dat (gel.cu:505-536): written by the output of a
+    time that is a multiple of 1000 (iteration 10^6 at dt = 1e-3) once an
+    earlier output has set the gel's flag; 1000 rows, one per ring slot,
+    `setw(9) << to_string(...)` of (time - 1000 + i, rm, Fn, Veln, vm, wm)
+    joined by six spaces.  The ring slots are the centre records of
+    iterations that are multiples of 1000 (gel.cu:706-713)."""
+    G = glsm
+    monkeypatch.chdir(tmp_path)
+    g, c, f = small_system(G)
+    for it in range(0, 6):
+        G.system_step([g], c, f, it)
+        g.record_center(1000 * (it + 1))  # ring slots 1..6 from the evolving state
+    ring = {k: g.read(k) for k in ("center_rm", "center_Fn", "center_Veln", "center_vm", "center_wm")}
+    g.write_files(0)        # first output: sets the flag
+    g.write_files(1000000)  # t = 1000: writes the ring
+    close_all(g, f, c)
+    rows = []
+    for i in range(1000):
+        vals = [*ring["center_rm"][i], *ring["center_Fn"][i], *ring["center_Veln"][i], ring["center_vm"][i],
+                ring["center_wm"][i]]
+        rows.append("      ".join(["%9s" % str(1000 - 1000 + i)] + ["%9s" % ts(v) for v in vals]) + "\n")
+    data = open(tmp_path / "gel1bodycenter.dat").read()
+    assert data == "".join(rows)
+    assert np.abs(ring["center_rm"][1:7]).min() > 0 and np.all(ring["center_rm"][7:] == 0)

@@ -407,7 +407,7 @@ def test_lennard_jones_pair(oracle, prec):
     r2 = d @ d
     sig2 = 1.5 ** 2
     s6 = (sig2 / r2) ** 3
-    delta = float(np.float32(1e-4))  # CouplerParams::delta is a float (coupling.cu:146)
+    delta = float(np.float32(1e-4))  # CouplerParams::delta is a float (coupling.cu:75)
     fac = 24 * delta * (2 * s6 * s6 - s6) / r2
     tol = 1e-13 if prec == 64 else 1e-6
     assert np.abs(Fr[0] - fac * d).max() < tol * abs(fac)
