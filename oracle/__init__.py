@@ -126,6 +126,8 @@ def lib():
             "oracle_delta4_f64": (d, [d]),
             "oracle_delta4_f32": (C.c_float, [C.c_float]),
             "oracle_lattice_tables": (None, [C.POINTER(i), C.POINTER(d), C.POINTER(i)]),
+            "oracle_gel_phase": (i, [vp, i]),
+            "oracle_solve_bilinear": (i, [C.POINTER(d), d, C.POINTER(d), C.POINTER(d)]),
             "oracle_fu": (d, [d, d, d, d]),
             "oracle_fv": (d, [d, d, d]),
         }
@@ -191,6 +193,18 @@ class Gel:
 
     def set_chemistry_rates(self, f, ep):
         lib().oracle_gel_set_chemistry_rates(self.h, float(f), float(ep))
+
+    PHASES = ["service_nodes", "element_properties", "chem_boundary", "unnorm", "pressure", "nodes_velocity",
+              "nodes_position", "terms", "chem", "set_zero"]
+
+    def phase(self, name):
+        """Run one phase of the gel step alone (calServiceNodesPositionD,
+        calElementPropertiesD, calChemBoundaryD, calUnnormD, calPressureD,
+        calNodesVelocityD, calInternalNodesPositionD, calTermsD, calChemD,
+        setZero)."""
+        rc = lib().oracle_gel_phase(self.h, self.PHASES.index(name))
+        if rc != 0:
+            raise RuntimeError(f"oracle_gel_phase({name}) -> {rc}")
 
     def steady_state(self):
         u, v, w = C.c_double(), C.c_double(), C.c_double()
@@ -336,3 +350,14 @@ def fu(u, v, w, f=0.9):
 
 def fv(u, v, w):
     return lib().oracle_fv(u, v, w)
+
+
+def solve_bilinear(corners, v_iso):
+    """solveBilinear (gel_kernels.cu:165-214): corners = (A, B, C, D) of the
+    first field at (0,0), (1,0), (0,1), (1,1), then (E, F, G, H) of the
+    second.  Returns (x, y) in the unit square where both bilinear
+    interpolants equal v_iso, or None."""
+    a = (C.c_double * 8)(*[float(v) for v in corners])
+    x, y = C.c_double(), C.c_double()
+    ok = lib().oracle_solve_bilinear(a, float(v_iso), C.byref(x), C.byref(y))
+    return (x.value, y.value) if ok else None

@@ -300,6 +300,30 @@ void oracle_lattice_tables(int* c57, double* w19, int* opp19) {
     w19[q] = q == 0 ? 1.0 / 3.0 : (q < 7 ? 1.0 / 18.0 : 1.0 / 36.0);
   }
 }
+// one phase of the gel step, for pinning the phases one at a time:
+// 0 serviceNodes, 1 elementProperties, 2 chemBoundary, 3 unnorm,
+// 4 pressure, 5 nodesVelocity, 6 nodesPosition, 7 terms, 8 chem, 9 setZero
+int oracle_gel_phase(void* gv, int phase) {
+  Gel* g = static_cast<Gel*>(gv);
+  switch (phase) {
+    case 0: g->serviceNodes(); break;
+    case 1: g->elementProperties(); break;
+    case 2: g->chemBoundary(); break;
+    case 3: g->unnorm(); break;
+    case 4: g->pressure(); break;
+    case 5: g->nodesVelocity(); break;
+    case 6: g->nodesPosition(); break;
+    case 7: g->terms(); break;
+    case 8: g->chem(); break;
+    case 9: g->setZero(); break;
+    default: return GLSM_EINVAL;
+  }
+  g->snapshotToHost();
+  return GLSM_OK;
+}
+int oracle_solve_bilinear(const double* AH, double v_iso, double* xb, double* yb) {
+  return Gel::solveBilinear(AH[0], AH[1], AH[2], AH[3], AH[4], AH[5], AH[6], AH[7], v_iso, *xb, *yb) ? 1 : 0;
+}
 double oracle_fu(double u, double v, double w, double f) { return Gel::fu(u, v, w, f, 0); }
 double oracle_fv(double u, double v, double w) { return Gel::fv(u, v, w, 0); }
 

@@ -458,3 +458,220 @@ def test_workload_generators():
                 assert np.any(np.abs(np.array(pa) - np.array(pb)) >= (na + nb) / 2 + 1 - 1e-9)
     rates = np.array([(f, ep) for i in inst for _, _, f, ep in i.gels])
     assert np.all(np.abs(rates[:, 0] / 0.9 - 1) <= 0.2) and np.all(np.abs(rates[:, 1] / 0.3 - 1) <= 0.2)
+
+
+# --------------------------------------------------------------- remaining gel laws
+GEL_CONST = dict(FA0=0.139, CH0=0.338, CH1=0.518, CHS=0.1, C0=1.3e-3, c0_bis=5e-4, b=0.01, AZ0=100.0)  # gel.cu:633-652
+
+
+def tensor_grid(g, maps):
+    """Node positions (all (L+2)^3 nodes, ghosts included, x fastest) of the
+    tensor-product grid x = X(i), y = Y(j), z = Z(k)."""
+    L = g.node_grid
+    k, j, i = np.meshgrid(np.arange(L[2] + 2), np.arange(L[1] + 2), np.arange(L[0] + 2), indexing="ij")
+    return np.stack([maps[0](i), maps[1](j), maps[2](k)], -1).reshape(-1, 3).astype(np.float64)
+
+
+def interior_elements(g):
+    L = g.node_grid
+    k, j, i = np.meshgrid(np.arange(L[2] + 1), np.arange(L[1] + 1), np.arange(L[0] + 1), indexing="ij")
+    inside = (i >= 1) & (i < L[0]) & (j >= 1) & (j < L[1]) & (k >= 1) & (k < L[2])
+    return inside.ravel()
+
+
+def test_T2_exact_for_quadratic_u_and_linear_w_on_a_nonuniform_grid(oracle):
+    """calTermsD's T2 = -grad w . grad u + (1 - w) lap u (gel_kernels.cu:680-702)
+    uses three-point derivatives on the unequal centroid spacings to both
+    neighbours; those are exact for quadratics.  On a tensor grid stretched
+    differently along each axis, with u quadratic (cross term included) and
+    w linear in the centroid coordinates, T2 equals the analytic value at
+    every interior element.  Swapping the two spacings, a sign, or (1 - w)
+    for w fails this."""
+    g = oracle.Gel((6, 5, 4), (10, 10, 10))
+    maps = (lambda i: i + 0.15 * i ** 2, lambda j: 1.3 * j + 0.05 * j ** 2, lambda k: 0.9 * k - 0.02 * k ** 2)
+    g.write("rn", tensor_grid(g, maps))
+    g.phase("element_properties")
+    rm = g.read("rm")
+    X, Y, Z = rm[:, 0], rm[:, 1], rm[:, 2]
+    gw = np.array([0.01, -0.02, 0.015])
+    w = 0.14 + gw[0] * X + gw[1] * Y + gw[2] * Z
+    u = 0.1 + 0.02 * X - 0.01 * Y + 0.03 * Z + 0.004 * X ** 2 - 0.003 * Y ** 2 + 0.002 * Z ** 2 + 0.001 * X * Y
+    g.write("wm", w)
+    g.write("um_norm", u)
+    g.phase("terms")
+    T2 = g.read("T2")
+    du = np.stack([0.02 + 0.008 * X + 0.001 * Y, -0.01 - 0.006 * Y + 0.001 * X, 0.03 + 0.004 * Z], -1)
+    lap = 0.008 - 0.006 + 0.004
+    expect = -(du @ gw) + (1 - w) * lap
+    m = interior_elements(g)
+    assert np.abs(np.diff(X.reshape(g.node_grid[2] + 1, g.node_grid[1] + 1, -1)[1, 1])).std() > 0.1  # unequal spacing
+    # rounding of the three-point sums of O(1) values over O(1) spacings
+    assert np.abs(T2[m] - expect[m]).max() < 1e-14
+    assert np.abs(expect[m]).min() > 1e-3
+
+
+def pm_series(w, v, gel_type):
+    """Osmotic pressure of the gLSM model, -(w + ln(1 - w) + chi(w) w^2) +
+    c0 w / (2 phi0) [+ chi* w v], chi(w) = CH0 + CH1 w, evaluated through
+    the power series -ln(1 - w) = sum w^k / k (independent of a log call)."""
+    k = GEL_CONST
+    s = (0.5 - k["CH0"]) * w ** 2 + (1 / 3 - k["CH1"]) * w ** 3
+    s = s + sum(w ** n / n for n in range(4, 200))
+    if gel_type == 1:
+        return s + k["C0"] * w / (2 * k["FA0"]) + k["CHS"] * w * v
+    return s + (k["c0_bis"] + k["b"] * v) * w / (2 * k["FA0"])
+
+
+@pytest.mark.parametrize("gel_type", [1, 2])
+def test_osmotic_pressure_law(oracle, gel_type):
+    """calPressureD (gel_kernels.cu:496-516) against the model's pressure
+    law expanded as a power series in the polymer fraction w."""
+    g = oracle.Gel((4, 4, 4), (10, 10, 10), gel_type=gel_type)
+    rng = np.random.default_rng(4)
+    nE = g.num_elements
+    w = rng.uniform(0.02, 0.3, nE)
+    v = rng.uniform(0.0, 0.5, nE)
+    g.write("wm", w)
+    g.write("vm", v)
+    g.phase("pressure")
+    m = interior_elements(g)
+    pm = g.read("pm")[m]
+    expect = pm_series(w[m], v[m], gel_type)
+    assert np.abs(pm - expect).max() < 1e-14
+    assert np.abs(expect).max() > 1e-3
+
+
+def test_node_velocity_mobility_and_volume_balance(oracle):
+    """calNodesVelocityD (gel_kernels.cu:518-598): the polymer velocity is the
+    node force times the model's mobility M(phi) = 8 AZ0 (1 - phi)
+    (phi0 / phi)^(1/2) (unit element volume), with phi the mean polymer
+    fraction of the node's eight elements, and the solvent velocity closes
+    the volume balance phi v_p + (1 - phi) v_s = 0.  For gel type 1 the
+    node forces of the whole gel add to zero (pair springs act equally on
+    both ends; a pressure acts on closed element surfaces) for any
+    deformation and any pressure field."""
+    g = oracle.Gel((4, 5, 3), (10, 10, 10))
+    rng = np.random.default_rng(6)
+    rn = g.read("rn")
+    g.write("rn", rn + 0.08 * rng.standard_normal(rn.shape))
+    g.phase("element_properties")
+    L = g.node_grid
+    w = rng.uniform(0.08, 0.3, g.num_elements)
+    g.write("wm", w)
+    g.write("vm", rng.uniform(0.0, 0.3, g.num_elements))
+    g.phase("pressure")
+    g.phase("nodes_velocity")
+    Fn, Vn, Vs = (g.read(k).reshape(L[2] + 2, L[1] + 2, L[0] + 2, 3)[1:-1, 1:-1, 1:-1].reshape(-1, 3)
+                  for k in ("Fn", "Veln", "Vels"))
+    we = w.reshape(L[2] + 1, L[1] + 1, L[0] + 1)
+    phi = sum(we[1 - c:L[2] + 1 - c, 1 - b:L[1] + 1 - b, 1 - a:L[0] + 1 - a]
+              for a in (0, 1) for b in (0, 1) for c in (0, 1)).reshape(-1) / 8
+    M = 8 * GEL_CONST["AZ0"] * (1 - phi) * np.sqrt(GEL_CONST["FA0"] / phi)
+    assert np.abs(Vn - M[:, None] * Fn).max() < 1e-12 * np.abs(Vn).max()
+    assert np.abs(phi[:, None] * Vn + (1 - phi[:, None]) * Vs).max() < 1e-13 * np.abs(Vn).max()
+    assert np.abs(Fn.sum(0)).max() < 1e-13 * np.abs(Fn).sum()
+    assert np.abs(Fn).max() > 1e-4
+
+
+def bilinear(c, x, y):
+    return c[0] * (1 - x) * (1 - y) + c[1] * x * (1 - y) + c[2] * (1 - x) * y + c[3] * x * y
+
+
+def roots_in_square(cu, cv, v_iso):
+    """All (x, y) in [0,1]^2 with both bilinear forms equal to v_iso, by
+    Newton iteration from a grid of starting points (independent of the
+    elimination the filament code uses)."""
+    found = []
+    for x0 in np.linspace(-0.1, 1.1, 7):
+        for y0 in np.linspace(-0.1, 1.1, 7):
+            x, y = x0, y0
+            for _ in range(40):
+                F = np.array([bilinear(cu, x, y) - v_iso, bilinear(cv, x, y) - v_iso])
+                J = np.array([[(cu[1] - cu[0]) * (1 - y) + (cu[3] - cu[2]) * y, (cu[2] - cu[0]) * (1 - x) + (cu[3] - cu[1]) * x],
+                              [(cv[1] - cv[0]) * (1 - y) + (cv[3] - cv[2]) * y, (cv[2] - cv[0]) * (1 - x) + (cv[3] - cv[1]) * x]])
+                if abs(np.linalg.det(J)) < 1e-14:
+                    break
+                x, y = np.array([x, y]) - np.linalg.solve(J, F)
+            if abs(bilinear(cu, x, y) - v_iso) < 1e-12 and abs(bilinear(cv, x, y) - v_iso) < 1e-12 \
+                    and -1e-9 <= x <= 1 + 1e-9 and -1e-9 <= y <= 1 + 1e-9:
+                if all(abs(x - a) + abs(y - b) > 1e-7 for a, b in found):
+                    found.append((x, y))
+    return found
+
+
+def test_solve_bilinear_finds_the_iso_line_crossing(oracle):
+    """solveBilinear (gel_kernels.cu:165-214): patches built around a chosen
+    crossing of the u = 0.15 and v = 0.15 iso-lines return a point where both
+    bilinear interpolants equal 0.15, inside the cell, and it is the crossing
+    Newton's method finds when that crossing is the only one; patches with no
+    crossing return none."""
+    rng = np.random.default_rng(12)
+    v_iso, unique, checked = 0.15, 0, 0
+    for _ in range(120):
+        xs, ys = rng.uniform(0.05, 0.95, 2)
+        au, av = rng.normal(0, 1, 3), rng.normal(0, 1, 3)  # (x, y, xy) slopes of each field
+        cu0 = v_iso - (au[0] * xs + au[1] * ys + au[2] * xs * ys)
+        cv0 = v_iso - (av[0] * xs + av[1] * ys + av[2] * xs * ys)
+        cu = [cu0, cu0 + au[0], cu0 + au[1], cu0 + au[0] + au[1] + au[2]]
+        cv = [cv0, cv0 + av[0], cv0 + av[1], cv0 + av[0] + av[1] + av[2]]
+        if abs(au[1] + au[2] * xs) < 0.05:  # the code solves y from the first field
+            continue
+        got = oracle.solve_bilinear(cu + cv, v_iso)
+        assert got is not None
+        x, y = got
+        assert 0 <= x <= 1 and 0 <= y <= 1
+        assert abs(bilinear(cu, x, y) - v_iso) < 1e-9 and abs(bilinear(cv, x, y) - v_iso) < 1e-9
+        roots = roots_in_square(cu, cv, v_iso)
+        checked += 1
+        if len(roots) == 1:
+            unique += 1
+            assert abs(x - roots[0][0]) < 1e-8 and abs(y - roots[0][1]) < 1e-8
+    assert checked > 80 and unique > 40
+    # both fields above the level everywhere: no crossing
+    assert oracle.solve_bilinear([0.3, 0.4, 0.5, 0.6, 0.2, 0.25, 0.3, 0.35], v_iso) is None
+    # the first field independent of y: the crossing (0.5, 0.5) exists, but y
+    # is solved from the first field (gel_kernels.cu:199-201), so none is
+    # reported -- the reference's behaviour, kept
+    assert oracle.solve_bilinear([0.1, 0.2, 0.1, 0.2, 0.1, 0.1, 0.2, 0.2], v_iso) is None
+
+
+# --------------------------------------------------------------- upwind advection
+def donor_cell(c, u, dt=1e-3, D=1.0):
+    """First-order upwind (donor-cell) finite volumes on a periodic grid with
+    face velocities the mean of the two cells, plus 7-point diffusion:
+    arrays indexed [z, y, x]."""
+    new = c.copy()
+    for ax, comp in ((2, 0), (1, 1), (0, 2)):
+        a = u[..., comp]
+        face = 0.5 * (a + np.roll(a, -1, axis=ax))            # between cell i and i+1
+        flux = np.where(face >= 0, face * c, face * np.roll(c, -1, axis=ax))
+        new -= dt * (flux - np.roll(flux, 1, axis=ax))
+        new += dt * D * (np.roll(c, 1, axis=ax) - 2 * c + np.roll(c, -1, axis=ax))
+    return new
+
+
+@pytest.mark.parametrize("case", ["+x", "-x", "+y", "-y", "+z", "-z", "mixed"])
+def test_upwind_advection_step(oracle, case):
+    """k_convection_diffusion (fluid_kernels.cu:177-242) on a periodic box with
+    no source: one step equals the donor-cell update -- flux through a face
+    = face velocity (the mean of the two cells) times the concentration of
+    the upstream cell.  Uniform velocities along each axis in both senses,
+    and a random velocity field of both signs."""
+    cp, fl = empty_fluid(oracle, (5, 4, 3), per=(1, 1, 1))
+    Nx, Ny, Nz = fl.grid
+    rng = np.random.default_rng(20)
+    c = rng.uniform(0, 1, (Nz, Ny, Nx))
+    u = np.zeros((Nz, Ny, Nx, 3))
+    if case == "mixed":
+        u = rng.uniform(-0.3, 0.3, u.shape)
+    else:
+        u[..., "xyz".index(case[1])] = 0.3 if case[0] == "+" else -0.3
+    fl.write("c", c.ravel())
+    fl.write("u", u.reshape(-1, 3))
+    fl.step_concentration(0)
+    got = fl.read("c").reshape(Nz, Ny, Nx)
+    expect = donor_cell(c, u)
+    assert np.abs(got - expect).max() < 1e-15
+    # the downwind choice differs by far more than that
+    assert np.abs(got - donor_cell(c, -u) - (donor_cell(c, u) - donor_cell(c, -u))).max() < 1e-15
+    assert np.abs(donor_cell(c, u) - donor_cell(c, -u)).max() > 1e-5
