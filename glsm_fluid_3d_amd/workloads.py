@@ -18,20 +18,22 @@ def round_to_odd(x: float) -> int:
     return int(2 * round((x - 1) / 2) + 1)
 
 
-def place_gels(rng, box: float, sizes, margin: float = 2.0, tries: int = 200):
+def place_gels(rng, box: float, sizes, margin: float = 2.0, tries: int = 200, room: float = 0.0):
     """Seeded centres for cube gels of the given element counts inside a
-    cubic box: at least `margin` from the walls and one element apart from
-    each other (their boxes separated along some axis).  A gel that finds no
-    place in `tries` draws is dropped.  Returns [(centre, n)]."""
+    cubic box: at least `margin` + room * n from the walls and
+    1 + 2 * room * n apart from each other (their boxes separated along some
+    axis).  `room` is the fraction of its edge a gel may grow by on each
+    side.  A gel that finds no place in `tries` draws is dropped.  Returns
+    [(centre, n)]."""
     out = []
     for n in sizes:
         half = n / 2.0
-        lo, hi = half + margin, box - half - margin
+        lo, hi = half + margin + room * n, box - half - margin - room * n
         if hi <= lo:
             continue
         for _ in range(tries):
             p = rng.uniform(lo, hi, 3)
-            if all(np.any(np.abs(p - q) >= half + m / 2.0 + 1.0) for q, m in out):
+            if all(np.any(np.abs(p - q) >= half + m / 2.0 + 1.0 + room * (n + m)) for q, m in out):
                 out.append((p, n))
                 break
     return out
@@ -82,12 +84,26 @@ class Job(Instance):
     extra: dict = field(default_factory=dict)
 
 
+# Over the serving horizons (up to 20000 steps) the BZ chemistry swells a gel
+# of the reference's constants until its element volume is ~6x the initial
+# one: each face moves out by up to about half the gel's edge.  A gel whose
+# surface reaches a wall puts IBM markers where their stencils are clipped
+# to a few cells; the spread force then concentrates (it is normalised by
+# the clipped weight sum, fluid_kernels.cu:300-348) and the explicit
+# coupling diverges once the ramped stiffness is large enough -- the same
+# happens in FP32 and FP64.  Serving jobs therefore keep that much room
+# around each gel.
+SWELL_ROOM = 0.5
+
+
 def poisson_jobs(seed=4, rate=20.0, duration=600.0, edge_range=(41, 161), gels_range=(1, 6), n=6,
-                 horizon_range=(1000, 20000), max_jobs=None):
+                 horizon_range=(1000, 20000), max_jobs=None, room=SWELL_ROOM):
     """The serving workload: Poisson arrivals (mean `rate` jobs/s) over
     `duration` seconds; lattice edge log-uniform in edge_range rounded to
     odd, 1-6 gels of n^3 elements at seeded non-overlapping positions with
-    the reference's rate constants, horizon uniform in horizon_range."""
+    room to swell (place_gels, `room`), the reference's rate constants,
+    horizon uniform in horizon_range.  A gel that finds no place in a small
+    box is dropped, so small lattices hold fewer gels."""
     rng = np.random.default_rng(seed)
     jobs = []
     t = 0.0
@@ -98,7 +114,7 @@ def poisson_jobs(seed=4, rate=20.0, duration=600.0, edge_range=(41, 161), gels_r
         edge = round_to_odd(np.exp(rng.uniform(np.log(edge_range[0]), np.log(edge_range[1]))))
         box = (edge - 1) * 0.5
         ng = int(rng.integers(gels_range[0], gels_range[1] + 1))
-        placed = place_gels(rng, box, [n] * ng)
+        placed = place_gels(rng, box, [n] * ng, room=room)
         gels = [(m, tuple(float(v) for v in p), F_REF, EP_REF) for p, m in placed]
         horizon = int(rng.integers(horizon_range[0], horizon_range[1] + 1))
         k = len(jobs)

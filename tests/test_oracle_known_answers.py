@@ -448,14 +448,15 @@ def test_workload_generators():
     assert abs(gaps.std() / gaps.mean() - 1.0) < 0.1                 # exponential gaps
     assert all(j.edge % 2 == 1 and 41 <= j.edge <= 161 and 1000 <= j.horizon <= 20000 for j in jobs)
     assert W.poisson_jobs(duration=5.0)[3].arrival == jobs[3].arrival  # seeded
-    for i in inst + jobs[:200]:
+    for i, room in [(x, 0.0) for x in inst] + [(x, W.SWELL_ROOM) for x in jobs[:200]]:
         assert 1 <= len(i.gels) <= 6
         for n, p, f, ep in i.gels:
-            assert all(n / 2 + 2 - 1e-9 <= x <= i.box - n / 2 - 2 + 1e-9 for x in p)
+            # walls: 2 + the room a serving gel may swell into (half its edge)
+            assert all(n / 2 + 2 + room * n - 1e-9 <= x <= i.box - n / 2 - 2 - room * n + 1e-9 for x in p)
         for a in range(len(i.gels)):
             for b in range(a):
                 (na, pa, _, _), (nb, pb, _, _) = i.gels[a], i.gels[b]
-                assert np.any(np.abs(np.array(pa) - np.array(pb)) >= (na + nb) / 2 + 1 - 1e-9)
+                assert np.any(np.abs(np.array(pa) - np.array(pb)) >= (na + nb) / 2 + 1 + room * (na + nb) - 1e-9)
     rates = np.array([(f, ep) for i in inst for _, _, f, ep in i.gels])
     assert np.all(np.abs(rates[:, 0] / 0.9 - 1) <= 0.2) and np.all(np.abs(rates[:, 1] / 0.3 - 1) <= 0.2)
 
