@@ -1,23 +1,27 @@
 #!/usr/bin/env python3
 """Benchmark of the gLSM_fluid_3D time step on one MI355X.
 
-Workload (one "step" = one gel time step: every gel's elasticity/chemistry,
-marker pack, 25 LBM sub-steps with immersed-boundary coupling, the
-concentration step, repulsion and scatter -- sim.cpp's loop body):
-  101^3 D3Q19 lattice (physical box 50^3, h = 0.5, walls) with the
-  concentration scalar, two 6x6x6-element gels at x = 1/3 and 2/3 of the box,
-  centred in y and z (sim.cpp's fractional placement), reference constants,
-  chemistry at steady state + splitmix64(seed 1) perturbation of 0.01,
-  FP32 arithmetic with FP32 deviation storage.
+Headline workload (one "step" = one gel time step: every gel's
+elasticity/chemistry, marker pack, 25 LBM sub-steps with immersed-boundary
+coupling, the concentration step, repulsion and scatter -- sim.cpp's loop
+body):
+  201^3 D3Q19 lattice (physical box 100^3, h = 0.5, walls) with the
+  concentration scalar, four 8x8x8-element gels at seeded non-overlapping
+  centres, reference constants, chemistry at steady state + splitmix64
+  perturbation of 0.01, FP32 arithmetic with FP32 deviation storage.  Both
+  distribution buffers (1.23 GB) are far larger than the 256 MB Infinity
+  Cache, so the stream/collide kernel runs from HBM.
 
 Metric: MLUPS = lattice cells x 25 sub-steps x steps / wall seconds / 1e6,
 inputs resident in HBM, no file output inside the timed region.
 
 Printed JSON line (contract): metric, value, unit, n_gpus, steps, warmup,
-ms_per_step, roofline (stream/collide kernel, live HIP-event timing),
-cpu_baseline (the FP32 OpenMP restatement on a bounded sample of the same
-workload), and extra lines for the 201^3 four-gel lattice in FP32 and FP16
-deviation storage.
+ms_per_step, roofline (stream/collide kernel, live HIP-event timing on the
+fluid's stream), cpu_baseline (the FP32 OpenMP restatement on a bounded
+sample of the same workload), and extra lines: the 201^3 lattice in FP16 and
+BF16 deviation storage, the 101^3 two-gel lattice (Infinity-Cache
+resident), the 64-instance batch in FP32 and FP16, and the Poisson serving
+stream (latency percentiles only once at least 20 jobs have completed).
 """
 import argparse
 import json
@@ -142,12 +146,15 @@ def run_slabs(G, storage="f32", parts=8, steps=40):
             x.close()
 
 
-def run_serve(G, seconds, outputs=True, seed=4, rate=20.0, drain=False):
-    """The serving workload for `seconds` of wall time: Poisson arrivals at
-    `rate` jobs/s (20 by default), edge 41-161, 1-6 gels of 6^3, horizons
-    1000-20000 steps, FP16 storage, .dat output streamed asynchronously (into
-    a scratch directory removed afterwards).  With `drain`, arrivals stop
-    after `seconds` and the server runs until every job has completed."""
+def run_serve(G, seconds, outputs=True, seed=4, rate=20.0, drain=False, min_completed=None):
+    """The serving workload: Poisson arrivals at `rate` jobs/s (20 by
+    default), edge 41-161, 1-6 gels of 6^3, horizons 1000-20000 steps, FP16
+    storage, .dat output streamed asynchronously (into a scratch directory
+    removed afterwards).  Runs for `seconds` of wall time, or, with
+    min_completed, until that many jobs have completed (at most `seconds`).
+    With `drain`, arrivals stop after `seconds` and the server runs until
+    every job has completed.  Latency percentiles are reported only over at
+    least 20 completed jobs."""
     import shutil
     import tempfile
 
@@ -158,13 +165,19 @@ def run_serve(G, seconds, outputs=True, seed=4, rate=20.0, drain=False):
     root = tempfile.mkdtemp(prefix="glsm_serve_") if outputs else None
     srv = S.Server(G, storage="f16", out_root=root, keep_output=False, writers=8)
     try:
-        stats = srv.run(jobs, duration=None if drain else seconds, progress_every=30.0)
+        stats = srv.run(jobs, duration=None if drain else seconds, progress_every=30.0,
+                        until_completed=None if drain else min_completed)
     finally:
         srv.close()
         if root:
             shutil.rmtree(root, ignore_errors=True)
-    stats["arrived"] = len(jobs)
-    stats["offered_MLUPS"] = round(sum(j.cells * 25 * j.horizon for j in jobs) / seconds / 1e6, 1)
+    arrived = [j for j in jobs if j.arrival <= stats["elapsed_s"]]
+    stats["arrived"] = len(arrived)
+    stats["offered_MLUPS"] = round(sum(j.cells * 25 * j.horizon for j in arrived) / stats["elapsed_s"] / 1e6, 1)
+    if stats["completed"] < 20:
+        for k in ("latency_p50_s", "latency_p99_s", "queue_wait_p50_s", "queue_wait_p99_s"):
+            stats[k] = None
+        stats["latency_note"] = f"only {stats['completed']} jobs completed: percentiles need at least 20"
     return stats
 
 
@@ -202,12 +215,40 @@ def load_traffic(name):
         return None
 
 
-def cpu_baseline(target_s):
-    import oracle as O
+def host_cpu():
+    """(model name, physical cores) of this host, from lscpu."""
+    import subprocess
 
-    threads = min(16, os.cpu_count() or 1)
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            info[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    try:
+        cores = int(info.get("Core(s) per socket", "0")) * int(info.get("Socket(s)", "0"))
+    except ValueError:
+        cores = 0
+    return info.get("Model name", "unknown"), cores or (os.cpu_count() or 1)
+
+
+def cpu_baseline(target_s):
+    """The FP32 OpenMP restatement (oracle/) on the headline workload: one
+    untimed step, then as many timed steps as fit in about target_s
+    seconds.  Threads: the CPU share this process may use ($OMP_NUM_THREADS,
+    else all physical cores), bound one per core."""
+    model, phys = host_cpu()
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or phys
+    threads = max(1, min(threads, phys))
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
+    import oracle as O  # loads libgomp with the binding above
+
     O.set_threads(threads)
-    gels, c, f = cube101_two_gels(O, prec=32)
+    gels, c, f = cube201_four_gels_oracle(O)
     N = f.N
     t0 = time.perf_counter()
     O.system_step(gels, c, f, 0)
@@ -218,22 +259,33 @@ def cpu_baseline(target_s):
         O.system_step(gels, c, f, it)
     dt = time.perf_counter() - t0
     return {"value": round(N * NSUB * steps / dt / 1e6, 2), "unit": "MLUPS", "cores": threads, "kind": "port",
-            "sample": f"FP32 OpenMP restatement (oracle/), same 101^3 two-gel workload, steps 1..{steps} "
-                      f"after one untimed step ({N * NSUB * steps / 1e6:.0f} M lattice updates, {dt:.1f} s)"}
+            "cpu_model": model, "physical_cores": phys,
+            "binding": f"OMP_PROC_BIND={os.environ['OMP_PROC_BIND']} OMP_PLACES={os.environ['OMP_PLACES']}",
+            "sample": f"FP32 OpenMP restatement (oracle/), the same 201^3 four-gel workload, steps 1..{steps} after "
+                      f"one untimed step ({N * NSUB * steps / 1e6:.0f} M lattice updates, {dt:.1f} s)"}
+
+
+def cube201_four_gels_oracle(O):
+    from glsm_fluid_3d_amd import workloads as W
+
+    return W.four_gels_201(O, prec=32)
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10000)
-    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=5000)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--storage", default="f32", choices=["f32", "f32full", "f16", "bf16"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--extra-steps", type=int, default=400)
     ap.add_argument("--batch-steps", type=int, default=300)
-    ap.add_argument("--serve-seconds", type=float, default=45.0)
+    ap.add_argument("--serve-seconds", type=float, default=300.0,
+                    help="serving window; the default run stops once 20 jobs have completed")
+    ap.add_argument("--serve-completed", type=int, default=20,
+                    help="stop the serving leg once this many jobs have completed (0 = run the whole window)")
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrival rate of the serving workload")
     ap.add_argument("--serve-drain", action="store_true",
                     help="stop arrivals after --serve-seconds and run until every job has completed")
@@ -247,7 +299,6 @@ def main():
     args = ap.parse_args()
     if args.workload != "all":
         args.no_cpu_baseline = True
-        args.no_extra = args.workload == "101"
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the time step runs on a single "
               "device and multi-GPU runs are not part of this build.", file=sys.stderr)
@@ -263,7 +314,8 @@ def main():
                           "GBs": round(G.copy_bandwidth(nbytes, max(1, args.steps)), 1)}), flush=True)
         return
     if args.workload == "serve":
-        r = run_serve(G, args.serve_seconds, rate=args.serve_rate, drain=args.serve_drain)
+        r = run_serve(G, args.serve_seconds, rate=args.serve_rate, drain=args.serve_drain,
+                      min_completed=args.serve_completed or None)
         r["workload"] = (f"Poisson {args.serve_rate:g} jobs/s for {args.serve_seconds:.0f} s"
                          f"{' then drained' if args.serve_drain else ''}, f16, async .dat output")
         print(json.dumps(r), flush=True)
@@ -289,19 +341,21 @@ def main():
         r["workload"] = f"64 instances (edge 33-97, 1-4 gels of 4^3-8^3), {st}"
         print(json.dumps(r), flush=True)
         return
-    if args.workload.startswith("201"):
-        st = args.workload.split("-")[1]
-        gels, c, f = cube201_four_gels(G, st)
+    if args.workload in ("101", "201-f32", "201-f16", "201-bf16"):
+        st = args.storage if args.workload == "101" else args.workload.split("-")[1]
+        gels, c, f = (cube101_two_gels(G, storage=st) if args.workload == "101" else cube201_four_gels(G, st))
         dt2, ms2, n2 = run_gpu(G, gels, c, f, args.steps, args.warmup)
         v2 = f.N * f.nsub * args.steps / dt2 / 1e6
-        print(json.dumps({"workload": f"201^3, four 8^3 gels, {st}", "value": round(v2, 1), "unit": "MLUPS",
+        print(json.dumps({"workload": f"{args.workload} lattice, {st}", "value": round(v2, 1), "unit": "MLUPS",
                           "steps": args.steps, "ms_per_step": round(dt2 / args.steps * 1e3, 4),
                           "roofline": roofline(f.N, 2 * 19 * G.STORAGE_BYTES[st], ms2)}), flush=True)
         return
-    gels, c, f = cube101_two_gels(G, storage=args.storage)
+
+    # ---- headline: 201^3, four 8^3 gels, FP32 deviation storage, HBM-resident
+    gels, c, f = cube201_four_gels(G, args.storage)
     N = f.N
-    log(f"workload 101^3: N={N} nsub={f.nsub} markers={c.M} storage={args.storage}")
-    dt, lbm_ms, samples = run_gpu(G, gels, c, f, args.steps, args.warmup)
+    log(f"workload 201^3: N={N} nsub={f.nsub} markers={c.M} storage={args.storage}")
+    dt, lbm_ms, samples = run_gpu(G, gels, c, f, args.steps, args.warmup, every=1)
     mlups = N * f.nsub * args.steps / dt / 1e6
     bpc = 2 * 19 * G.STORAGE_BYTES[args.storage]
     line = {
@@ -310,15 +364,17 @@ def main():
         "ms_per_step": round(dt / args.steps * 1e3, 4), "gel_steps_per_s": round(args.steps / dt, 1),
         "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "101^3 D3Q19 + concentration, two 6x6x6-element gels at x=1/3,2/3 (sim.cpp "
-                               "placement), walls, seeded chemistry perturbation, 25 LBM sub-steps per gel step",
-                   "lattice": [101, 101, 101], "gels": 2, "markers": c.M, "nsub": f.nsub,
+        "config": {"workload": "201^3 D3Q19 + concentration, four 8x8x8-element gels at seeded non-overlapping "
+                               "centres, walls, seeded chemistry perturbation, 25 LBM sub-steps per gel step "
+                               "(HBM-resident: 1.23 GB of distributions)",
+                   "lattice": list(f.grid), "gels": len(gels), "markers": c.M, "nsub": f.nsub,
                    "storage": args.storage + (" (f - w_q deviations)" if args.storage != "f32full" else ""),
                    "parallelism": "single GPU"},
-        "roofline": roofline(N, bpc, lbm_ms, load_traffic(f"cube101_two_gels_{args.storage}")),
+        "roofline": roofline(N, bpc, lbm_ms, load_traffic(f"cube201_four_gels_{args.storage}")),
     }
     line["roofline"]["samples"] = samples
-    log(f"101^3: {mlups:.0f} MLUPS, {dt / args.steps * 1e3:.3f} ms/step, lbm kernel {lbm_ms * 1e3:.1f} us")
+    log(f"201^3: {mlups:.0f} MLUPS, {dt / args.steps * 1e3:.3f} ms/step, lbm kernel {lbm_ms * 1e3:.1f} us "
+        f"({samples} samples)")
     for x in gels + [c, f]:
         x.close()
     copy_gbs = G.copy_bandwidth(19 * 4 * 8120601, 20)
@@ -327,25 +383,39 @@ def main():
 
     if not args.no_extra:
         extra = {}
-        for st in ("f32", "f16"):
+        for st in ("f16", "bf16"):
             gels, c, f = cube201_four_gels(G, st)
             dt2, ms2, n2 = run_gpu(G, gels, c, f, args.extra_steps, 20)
             v2 = f.N * f.nsub * args.extra_steps / dt2 / 1e6
             extra[f"201cube_4gels_{st}"] = {
                 "value": round(v2, 1), "unit": "MLUPS", "steps": args.extra_steps,
-                "ms_per_step": round(dt2 / args.extra_steps * 1e3, 4),
+                "ms_per_step": round(dt2 / args.extra_steps * 1e3, 4), "vs_headline": round(v2 / mlups, 3),
                 "roofline": roofline(f.N, 2 * 19 * G.STORAGE_BYTES[st], ms2, load_traffic(f"cube201_four_gels_{st}"))}
-            log(f"201^3 {st}: {v2:.0f} MLUPS, lbm kernel {ms2 * 1e3:.1f} us")
+            log(f"201^3 {st}: {v2:.0f} MLUPS ({v2 / mlups:.2f}x), lbm kernel {ms2 * 1e3:.1f} us")
             for x in gels + [c, f]:
                 x.close()
+        gels, c, f = cube101_two_gels(G, storage="f32")
+        dt2, ms2, n2 = run_gpu(G, gels, c, f, 4 * args.extra_steps, 20)
+        v2 = f.N * f.nsub * 4 * args.extra_steps / dt2 / 1e6
+        extra["101cube_2gels_f32"] = {
+            "value": round(v2, 1), "unit": "MLUPS", "steps": 4 * args.extra_steps,
+            "ms_per_step": round(dt2 / (4 * args.extra_steps) * 1e3, 4),
+            "note": "reference default scene (sim.cpp); both distribution buffers (157 MB) fit in the 256 MB "
+                    "Infinity Cache, so its roofline fraction is cache-assisted, not an HBM figure",
+            "roofline": roofline(f.N, 152, ms2, load_traffic("cube101_two_gels_f32"))}
+        log(f"101^3 f32: {v2:.0f} MLUPS, lbm kernel {ms2 * 1e3:.1f} us")
+        for x in gels + [c, f]:
+            x.close()
         for st in ("f32", "f16"):
             r = run_batch(G, st, args.batch_steps, 20)
+            r["vs_headline"] = round(r["value"] / mlups, 3)
             extra[f"batch64_{st}"] = r
-            log(f"batch of 64 {st}: {r['value']:.0f} MLUPS, lbm kernel {r['roofline']['mean_launch_ms'] * 1e3:.1f} us, "
-                f"launch {r['launch']}")
-        r = run_serve(G, args.serve_seconds)
+            log(f"batch of 64 {st}: {r['value']:.0f} MLUPS ({r['vs_headline']:.2f}x), lbm kernel "
+                f"{r['roofline']['mean_launch_ms'] * 1e3:.1f} us, launch {r['launch']}")
+        r = run_serve(G, args.serve_seconds, min_completed=args.serve_completed or None)
+        r["vs_headline"] = round(r["mlups"] / mlups, 3) if r.get("mlups") else None
         extra["serve_poisson20_f16"] = r
-        log(f"serving {args.serve_seconds:.0f} s: {r['completed']} done, p50 {r['latency_p50_s']} s, "
+        log(f"serving {r['elapsed_s']:.0f} s: {r['completed']} done, p50 {r['latency_p50_s']} s, "
             f"p99 {r['latency_p99_s']} s, {r['mlups']:.0f} MLUPS")
         line["extra"] = extra
 

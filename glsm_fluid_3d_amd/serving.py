@@ -225,11 +225,13 @@ class Server:
                 self._release(rec, now, failure)
 
     # ---- the loop ----------------------------------------------------------
-    def run(self, jobs, duration=None, clock=None, idle_sleep=0.002, progress_every=None, log=None):
+    def run(self, jobs, duration=None, clock=None, idle_sleep=0.002, progress_every=None, log=None,
+            until_completed=None):
         """Serve `jobs` (arrival times relative to the start) until all are
-        done or `duration` seconds have passed.  `clock` () -> seconds since
-        start (default: wall time).  With progress_every, `log` (default:
-        stderr) gets a status line every that many seconds."""
+        done, `duration` seconds have passed, or (with until_completed) that
+        many jobs have completed.  `clock` () -> seconds since start
+        (default: wall time).  With progress_every, `log` (default: stderr)
+        gets a status line every that many seconds."""
         import sys
 
         log = log or (lambda m: print(m, file=sys.stderr, flush=True))
@@ -241,6 +243,8 @@ class Server:
         while True:
             now = now_fn()
             if duration is not None and now >= duration:
+                break
+            if until_completed is not None and len(self.done) >= until_completed:
                 break
             while pending and pending[0].arrival <= now:
                 self.submit(pending.popleft())
