@@ -76,6 +76,7 @@ SIGNATURES = {
     "glsm_fluid_write_files": (_i, [_vp, _i]),
     "glsm_fluid_save_state": (_i, [_vp, C.c_char_p]),
     "glsm_fluid_conservation": (_i, [_vp, C.POINTER(_d)]),
+    "glsm_fluid_conservation_static": (_i, [_vp, C.POINTER(_d)]),
     "glsm_fluid_load_state": (_i, [_vp, C.c_char_p]),
     "glsm_fluid_check_finite": (_i, [_vp, C.POINTER(C.c_longlong), C.POINTER(_d), C.POINTER(_d)]),
     "glsm_fluid_set_periodic": (_i, [_vp, _i, _i, _i]),

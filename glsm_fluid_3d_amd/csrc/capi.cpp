@@ -345,6 +345,7 @@ int glsm_init(int device) {
     REQUIRE(device >= 0, "device must be >= 0");
     glsm::set_device(device);
     (void)glsm::stream();
+    (void)glsm::static_scratch();  // the allocation-free entry point's scratch on this device
     return GLSM_OK;
   });
 }
@@ -574,6 +575,15 @@ int glsm_fluid_conservation(glsm_fluid* f, double out[4]) {
   return guarded([&] {
     REQUIRE(out, "null output pointer");
     F(f)->conservation(out);
+    return GLSM_OK;
+  });
+}
+int glsm_fluid_conservation_static(glsm_fluid* f, double out[4]) {
+  return guarded([&] {
+    REQUIRE(out, "null output pointer");
+    Fluid* p = F(f);
+    glsm::conservation_static(p->storage(), glsm::fluid_geom(*p), glsm::fluid_lbm_params(*p), p->d_f, p->plane, out,
+                              p->fluid_stream);
     return GLSM_OK;
   });
 }

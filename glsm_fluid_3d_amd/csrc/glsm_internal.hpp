@@ -238,6 +238,15 @@ void launch_lbm_health(int storage, const LatticeGeom& g, const LbmParams& p, co
 // p[0..n) = 0, and *out += number of non-finite values of a[0..n)
 void launch_zero_u64(unsigned long long* p, int n, hipStream_t s);
 void launch_count_nonfinite(const double* a, size_t n, unsigned long long* out, hipStream_t s);
+// The FP64 conservation sums of launch_lbm_moments with their scratch in a
+// static device array instead of an allocation (lattices of at most
+// static_scratch_max_cells() cells); results copied to host_out4, stream
+// synchronised.  static_scratch() looks up the array's address on the
+// current device (once per device).
+double* static_scratch();
+long static_scratch_max_cells();
+void conservation_static(int storage, const LatticeGeom& g, const LbmParams& p, const void* fpost, size_t plane,
+                         double* host_out4, hipStream_t s);
 // rho - 1 per cell (FP64) of the pre-collision populations of the next sub-step
 void launch_lbm_density(int storage, const LatticeGeom& g, const LbmParams& p, const void* fpost, size_t plane,
                         double* out, hipStream_t s);

@@ -31,6 +31,8 @@
 //   F16_DEV  : d * 2^s in binary16 (s = scale exponent, default 20);
 //   BF16_DEV : d in bfloat16 (round to nearest even).
 #include <cstddef>
+#include <map>
+#include <mutex>
 #include <type_traits>
 
 #include <hip/hip_ext.h>
@@ -796,6 +798,57 @@ void launch_lbm_density(int storage, const LatticeGeom& g, const LbmParams& p, c
   GLSM_STORAGE_DISPATCH(storage, (k_lbm_density<S><<<blocks_for(g.N), kThreads, 0, s>>>(
                                      g, p, static_cast<const typename Store<S>::T*>(fpost), (unsigned)plane, out)));
   GLSM_LAUNCH_CHECK();
+}
+
+// ---- allocation-free FP64 conservation sums ----
+// Scratch of conservation_static(): the four FP64 partial sums of up to
+// kStaticBlocks 256-cell blocks, then the four results -- 4 MiB in a
+// file-scope device array that every device the library runs on holds until
+// the process ends.  In host code `scratch` names a host-side stand-in, not
+// device memory: its device address is looked up once per device
+// (hipGetSymbolAddress, with that device current) and only that address is
+// passed to kernels and copies.  Every entry a call reads was written by the
+// same call first (each block stores all four of its partials), so nothing
+// needs clearing; calls that overlap on one device share the array and race.
+namespace {
+constexpr int kStaticBlocks = 131072;  // lattices of up to 33,554,432 cells
+__device__ __align__(256) double scratch[4 * kStaticBlocks + 4];
+std::mutex g_scratch_mu;
+std::map<int, double*> g_scratch_ptr;  // device -> device address of `scratch`
+}  // namespace
+
+double* static_scratch() {
+  int dev = 0;
+  GLSM_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch_ptr.find(dev);
+  if (it != g_scratch_ptr.end()) return it->second;
+  void* ptr = nullptr;
+  GLSM_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  if (!ptr) throw std::runtime_error("hipGetSymbolAddress returned no address for the conservation scratch");
+  g_scratch_ptr[dev] = static_cast<double*>(ptr);
+  return static_cast<double*>(ptr);
+}
+
+long static_scratch_max_cells() { return (long)kStaticBlocks * kThreads; }
+
+void conservation_static(int storage, const LatticeGeom& g, const LbmParams& p, const void* fpost, size_t plane,
+                         double* host_out4, hipStream_t s) {
+  const int nb = blocks_for(g.N);
+  if (nb > kStaticBlocks)
+    throw std::invalid_argument("conservation without allocation: the lattice has " + std::to_string(g.N) +
+                                " cells, the static scratch holds at most " +
+                                std::to_string(static_scratch_max_cells()));
+  double* partial = static_scratch();
+  double* out4 = partial + 4ull * kStaticBlocks;
+  GLSM_STORAGE_DISPATCH(storage, (k_lbm_moments<S><<<nb, kThreads, 0, s>>>(
+                                     g, p, static_cast<const typename Store<S>::T*>(fpost), (unsigned)plane,
+                                     partial)));
+  GLSM_LAUNCH_CHECK();
+  k_reduce_partials<<<1, kThreads, 0, s>>>(partial, nb, out4);
+  GLSM_LAUNCH_CHECK();
+  GLSM_HIP(hipMemcpyAsync(host_out4, out4, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
+  GLSM_HIP(hipStreamSynchronize(s));
 }
 
 void launch_lbm_import(int storage, const LatticeGeom& g, const LbmParams& p, const float* in, void* fpost,

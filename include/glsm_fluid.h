@@ -153,6 +153,15 @@ int glsm_fluid_write_files(glsm_fluid* f, int iter);        /* fluid.cu:140-150 
    rho*u = sum_q f_q c_q as in k_macros (fluid_kernels.cu:98-111) without the
    F/2 term.  Per-cell sums in FP64, fixed-order block reduction. */
 int glsm_fluid_conservation(glsm_fluid* f, double out[4]);
+/* No counterpart in the reference.  glsm_fluid_conservation for callers that
+   cannot allocate device memory (code under HIP graph capture, tight serving
+   loops, harnesses that forbid hipMalloc): the same kernels and the same
+   bits, with the per-block FP64 partials in a static 4 MiB device array
+   instead of library-allocated memory.  No hipMalloc/hipFree or allocating
+   library call.  Lattices of at most 33,554,432 cells (GLSM_EINVAL beyond).
+   Every call on a device shares that device's array: calls that overlap on
+   one device (several host threads, or streams) race. */
+int glsm_fluid_conservation_static(glsm_fluid* f, double out[4]);
 /* Extension (failure detection): bad_cells = cells whose stored populations,
    velocity or concentration hold a non-finite value -- a binary16 deviation
    beyond the storage's range is stored as Inf and counts; max_abs_dev (may
