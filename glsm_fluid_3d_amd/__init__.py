@@ -98,8 +98,8 @@ SIGNATURES = {
     "glsm_batch_iteration": (_i, [_vp, _i]),
     "glsm_batch_step": (_i, [_vp, _i]),
     "glsm_batch_use_graph": (_i, [_vp, _i]),
-    "glsm_batch_set_launch": (_i, [_vp, _i, _i, _i]),
-    "glsm_batch_get_launch": (_i, [_vp, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i)]),
+    "glsm_batch_set_launch": (_i, [_vp, _i, _i, _i, _i, _i]),
+    "glsm_batch_get_launch": (_i, [_vp] + [C.POINTER(_i)] * 5),
     "glsm_batch_autotune": (_i, [_vp, _i, C.POINTER(_d)]),
     "glsm_batch_enable_lbm_timing": (_i, [_vp, _i]),
     "glsm_batch_lbm_timing": (_i, [_vp, C.POINTER(_d), C.POINTER(_i)]),
@@ -441,6 +441,7 @@ class Batch:
         h = C.c_void_p()
         _check(load().glsm_batch_create(STORAGE[storage], int(scale_exp), C.byref(h)), "glsm_batch_create")
         self.h = h.value
+        self.storage = storage
         self.members = {}
 
     def close(self):
@@ -477,18 +478,43 @@ class Batch:
         self.output = output
         _check(load().glsm_batch_set_output(self.h, output.h if output else None), "batch set_output")
 
-    def set_launch(self, nt, run, grid):
-        _check(load().glsm_batch_set_launch(self.h, int(nt), int(run), int(grid)), "batch set_launch")
+    def set_launch(self, nt, run, grid, wgb=1, chunk=1):
+        """Launch shape of the batched stream/collide kernel (see
+        glsm_batch_set_launch)."""
+        _check(load().glsm_batch_set_launch(self.h, int(nt), int(run), int(grid), int(wgb), int(chunk)),
+               "batch set_launch")
 
     def launch(self):
-        a, b, c = C.c_int(), C.c_int(), C.c_int()
-        _check(load().glsm_batch_get_launch(self.h, C.byref(a), C.byref(b), C.byref(c)), "batch launch")
-        return {"nt": a.value, "run": b.value, "grid": c.value}
+        v = [C.c_int() for _ in range(5)]
+        _check(load().glsm_batch_get_launch(self.h, *[C.byref(x) for x in v]), "batch launch")
+        return dict(zip(("nt", "run", "grid", "wgb", "chunk"), (x.value for x in v)))
 
-    def autotune(self, reps=5):
+    def autotune(self, reps=5, cache=None):
+        """Time every launch shape on the current state and keep the fastest;
+        returns (shape, mean launch ms).  With `cache` (a JSON file path, or
+        True for the default tune_cache_path()), a shape stored for the same
+        workload key -- storage, device, the members' lattices and marker
+        counts -- is reused without timing, and a new pick is stored."""
+        path = tune_cache_path() if cache is True else cache
+        key = self.workload_key() if path else None
+        if path:
+            hit = _tune_cache_get(path, key)
+            if hit is not None:
+                self.set_launch(**hit["launch"])
+                return self.launch(), hit["ms"]
         ms = C.c_double()
         _check(load().glsm_batch_autotune(self.h, int(reps), C.byref(ms)), "batch autotune")
+        if path:
+            _tune_cache_put(path, key, {"launch": self.launch(), "ms": ms.value})
         return self.launch(), ms.value
+
+    def workload_key(self):
+        """The key a tuned launch shape is cached under."""
+        import hashlib
+
+        members = sorted((f.grid, c.M) for _, c, f in self.members.values())
+        digest = hashlib.sha1(repr(members).encode()).hexdigest()[:16]
+        return f"{self.storage}|{device_name()}|{len(members)} members|{sum(f.N for _, _, f in self.members.values())} cells|{digest}"
 
     def enable_lbm_timing(self, every):
         _check(load().glsm_batch_enable_lbm_timing(self.h, int(every)), "batch enable_lbm_timing")
@@ -591,3 +617,44 @@ def device_memory():
     f, t = C.c_size_t(), C.c_size_t()
     _check(load().glsm_device_memory(C.byref(f), C.byref(t)), "glsm_device_memory")
     return f.value, t.value
+
+
+def device_name():
+    """The library build (target architecture) and the current device's
+    memory size: the device part of the tuning cache key."""
+    return f"{load().glsm_version().decode(errors='replace')} {device_memory()[1] >> 30} GiB"
+
+
+def tune_cache_path():
+    """$GLSM_TUNE_CACHE, else $XDG_CACHE_HOME (or ~/.cache)/glsm_fluid_3d_amd/launch_shapes.json."""
+    p = os.environ.get("GLSM_TUNE_CACHE")
+    if p:
+        return p
+    root = os.environ.get("XDG_CACHE_HOME") or os.path.join(os.path.expanduser("~"), ".cache")
+    return os.path.join(root, "glsm_fluid_3d_amd", "launch_shapes.json")
+
+
+def _tune_cache_get(path, key):
+    import json
+
+    try:
+        with open(path) as fh:
+            return json.load(fh).get(key)
+    except (OSError, ValueError):
+        return None
+
+
+def _tune_cache_put(path, key, value):
+    import json
+
+    try:
+        with open(path) as fh:
+            data = json.load(fh)
+    except (OSError, ValueError):
+        data = {}
+    data[key] = value
+    os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+    tmp = path + ".tmp"
+    with open(tmp, "w") as fh:
+        json.dump(data, fh, indent=1, sort_keys=True)
+    os.replace(tmp, path)

@@ -60,6 +60,8 @@ glsm::LbmLaunch to_internal(const Batch::Launch& l) {
   s.nt = l.nt;
   s.run = l.run;
   s.grid = l.grid;
+  s.wgb = l.wgb;
+  s.chunk = l.chunk;
   return s;
 }
 }  // namespace
@@ -136,6 +138,10 @@ void Batch::setLaunch(const Launch& l) {
   if (l.run != 0 && l.run != 4 && l.run != 16 && l.run != 64)
     throw std::invalid_argument("Batch::setLaunch: run must be 0, 4, 16 or 64");
   if (l.grid < 0) throw std::invalid_argument("Batch::setLaunch: negative grid");
+  if (l.wgb != 1 && l.wgb != 2 && l.wgb != 4) throw std::invalid_argument("Batch::setLaunch: wgb must be 1, 2 or 4");
+  if (l.chunk < 1 || l.chunk > 64) throw std::invalid_argument("Batch::setLaunch: chunk must be in [1, 64]");
+  if ((l.wgb != 1 || l.chunk != 1) && (l.grid != 0 || l.run == 64))
+    throw std::invalid_argument("Batch::setLaunch: work units of several blocks take grid 0 and run 0, 4 or 16");
   launch_ = l;
   launch_set_ = true;
   dropGraphs();
@@ -248,6 +254,8 @@ void Batch::rebuild() {
     launch_.nt = fbytes > (size_t)192 << 20 ? 1 : 0;
     launch_.run = launch_.nt ? 4 : 0;
     launch_.grid = 0;
+    launch_.wgb = 1;
+    launch_.chunk = 1;
   }
   GLSM_HIP(hipStreamSynchronize(s));
   dirty_ = false;
@@ -375,7 +383,12 @@ Batch::Launch Batch::autotune(int reps, double* best_ms) {
   std::vector<Launch> cand;
   for (int nt = 0; nt < 2; ++nt)
     for (int run : {0, 4, 16, 64})
-      for (int g : {0, 2, 4, 8, 16}) cand.push_back(Launch{nt, run, g ? g * cus : 0});
+      for (int g : {0, 2, 4, 8, 16}) cand.push_back(Launch{nt, run, g ? g * cus : 0, 1, 1});
+  for (int nt = 0; nt < 2; ++nt)
+    for (int run : {0, 4, 16}) {
+      for (int wgb : {2, 4}) cand.push_back(Launch{nt, run, 0, wgb, 1});
+      for (int chunk : {2, 4, 8}) cand.push_back(Launch{nt, run, 0, 1, chunk});
+    }
   hipEvent_t e0, e1;
   GLSM_HIP(hipEventCreate(&e0));
   GLSM_HIP(hipEventCreate(&e1));

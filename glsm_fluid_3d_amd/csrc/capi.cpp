@@ -746,19 +746,21 @@ int glsm_batch_use_graph(glsm_batch* b, int on) {
     return GLSM_OK;
   });
 }
-int glsm_batch_set_launch(glsm_batch* b, int nt, int run, int grid) {
+int glsm_batch_set_launch(glsm_batch* b, int nt, int run, int grid, int wgb, int chunk) {
   return guarded([&] {
-    B(b)->setLaunch(Batch::Launch{nt, run, grid});
+    B(b)->setLaunch(Batch::Launch{nt, run, grid, wgb, chunk});
     return GLSM_OK;
   });
 }
-int glsm_batch_get_launch(glsm_batch* b, int* nt, int* run, int* grid) {
+int glsm_batch_get_launch(glsm_batch* b, int* nt, int* run, int* grid, int* wgb, int* chunk) {
   return guarded([&] {
-    REQUIRE(nt && run && grid, "null output pointer");
+    REQUIRE(nt && run && grid && wgb && chunk, "null output pointer");
     const Batch::Launch l = B(b)->launch();
     *nt = l.nt;
     *run = l.run;
     *grid = l.grid;
+    *wgb = l.wgb;
+    *chunk = l.chunk;
     return GLSM_OK;
   });
 }

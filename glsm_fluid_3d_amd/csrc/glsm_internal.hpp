@@ -312,13 +312,17 @@ void launch_scalar_batch(const BatchTables& t, int cpar, hipStream_t s);
 
 
 // Launch shape of the batched stream/collide kernel (the autotuner's knobs):
-// non-temporal stores, XCD run length (0 = one contiguous run per XCD) and
-// the number of work-groups (0 = one per block; fewer = persistent
-// work-groups looping over the blocks).
+// non-temporal stores, XCD run length (0 = one contiguous run per XCD), the
+// number of work-groups (0 = one per work unit; fewer = persistent
+// work-groups looping over the units), 256-cell blocks stepped together by
+// one work-group (wgb: 256, 512 or 1024 threads) and consecutive rounds of
+// blocks per work-group (chunk: cells per thread).
 struct LbmLaunch {
   int nt = 0;
   int run = 0;
   int grid = 0;
+  int wgb = 1;
+  int chunk = 1;
 };
 // one sub-step of every member: f[sel] -> f[sel ^ 1]
 void launch_lbm_step_batch(int storage, const BatchTables& t, int sel, bool write_u_all, const LbmLaunch& shape,

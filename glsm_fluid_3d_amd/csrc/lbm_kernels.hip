@@ -327,8 +327,9 @@ __device__ __forceinline__ void lbm_block(const LatticeGeom& g, const LbmParams&
                                           const typename Store<S>::T* __restrict__ fin,
                                           typename Store<S>::T* __restrict__ fout, unsigned plane,
                                           float3* __restrict__ u, int write_u_all, const SpreadLists& sl,
-                                          const float3* __restrict__ Fl, int const_force, unsigned b) {
-  const unsigned i = b * kThreads + threadIdx.x;
+                                          const float3* __restrict__ Fl, int const_force, unsigned b,
+                                          unsigned lane) {
+  const unsigned i = b * kThreads + lane;
   const int flagged = gmem(sl.blkflag)[b];
   if (i >= (unsigned)g.N) return;
   int x, y, z;
@@ -394,7 +395,7 @@ __global__ void __launch_bounds__(kThreads, 8) k_lbm_step(LatticeGeom g, LbmPara
                                                        float3* __restrict__ u, int write_u_all, SpreadLists sl,
                                                        const float3* __restrict__ Fl, int const_force) {
   const unsigned b = xcd_block<NT ? 4 : 0>(blockIdx.x, gridDim.x);
-  lbm_block<S, NT, B32>(g, p, fin, fout, plane, u, write_u_all, sl, Fl, const_force, b);
+  lbm_block<S, NT, B32>(g, p, fin, fout, plane, u, write_u_all, sl, Fl, const_force, b, threadIdx.x);
 }
 
 // All members of a batch: the blocks of every lattice form one flattened
@@ -409,7 +410,30 @@ __global__ void __launch_bounds__(kThreads, 8) kb_lbm_step(BatchTables t, int se
     const BatchInst& I = t.inst[t.blk_inst[gb]];
     lbm_block<S, NT, true>(I.g, I.p, static_cast<const T*>(I.f[sel]), static_cast<T*>(I.f[sel ^ 1]),
                            (unsigned)I.plane, I.u, write_u_all, I.sl, I.m.Fdrag, I.const_force,
-                           gb - (unsigned)I.blk0);
+                           gb - (unsigned)I.blk0, threadIdx.x);
+  }
+}
+
+// The same in work units of WGB * chunk consecutive 256-cell blocks: a
+// work-group of WGB * 256 threads takes one unit, its WGB groups of four
+// waves each one block per round, for `chunk` rounds over consecutive
+// blocks (the shifted planes' edge lines of round r are read again in round
+// r + 1 by the same CU).  One work-group per unit, the XCD remap applied to
+// the unit index.
+template <int S, bool NT, int RUN, int WGB>
+__global__ void __launch_bounds__(kThreads * WGB, 8) kb_lbm_step_units(BatchTables t, int sel, int write_u_all,
+                                                                       int chunk) {
+  using T = typename Store<S>::T;
+  const unsigned per_unit = (unsigned)(WGB * chunk);
+  const unsigned nunits = ((unsigned)t.nblk + per_unit - 1) / per_unit;
+  const unsigned base = xcd_block<RUN>(blockIdx.x, nunits) * per_unit + threadIdx.x / kThreads;
+  for (int r = 0; r < chunk; ++r) {
+    const unsigned gb = base + (unsigned)(r * WGB);
+    if (gb >= (unsigned)t.nblk) break;
+    const BatchInst& I = t.inst[t.blk_inst[gb]];
+    lbm_block<S, NT, true>(I.g, I.p, static_cast<const T*>(I.f[sel]), static_cast<T*>(I.f[sel ^ 1]),
+                           (unsigned)I.plane, I.u, write_u_all, I.sl, I.m.Fdrag, I.const_force,
+                           gb - (unsigned)I.blk0, threadIdx.x % kThreads);
   }
 }
 
@@ -704,11 +728,43 @@ void launch_lbm_step(int storage, const LatticeGeom& g, const LbmParams& p, cons
     default: throw std::invalid_argument("unknown storage mode");   \
   }
 
+template <int S, int WGB>
+void step_batch_units(const BatchTables& t, int sel, int wu, const LbmLaunch& sh, hipStream_t s, hipEvent_t ev0,
+                      hipEvent_t ev1) {
+  const int per_unit = WGB * sh.chunk;
+  const int grid = (t.nblk + per_unit - 1) / per_unit;
+  auto go = [&](auto kernel) {
+    if (ev0)
+      hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(kThreads * WGB), 0, s, ev0, ev1, 0, t, sel, wu, sh.chunk);
+    else
+      kernel<<<grid, kThreads * WGB, 0, s>>>(t, sel, wu, sh.chunk);
+  };
+  switch (sh.run * 2 + (sh.nt ? 1 : 0)) {
+    case 0: go(kb_lbm_step_units<S, false, 0, WGB>); break;
+    case 1: go(kb_lbm_step_units<S, true, 0, WGB>); break;
+    case 8: go(kb_lbm_step_units<S, false, 4, WGB>); break;
+    case 9: go(kb_lbm_step_units<S, true, 4, WGB>); break;
+    case 32: go(kb_lbm_step_units<S, false, 16, WGB>); break;
+    case 33: go(kb_lbm_step_units<S, true, 16, WGB>); break;
+    default: throw std::invalid_argument("work units of several blocks take an XCD run length of 0, 4 or 16");
+  }
+}
+
 template <int S>
 void step_batch_impl(const BatchTables& t, int sel, bool write_u_all, const LbmLaunch& sh, hipStream_t s,
                      hipEvent_t ev0, hipEvent_t ev1) {
-  const int grid = sh.grid > 0 && sh.grid < t.nblk ? sh.grid : t.nblk;
   const int wu = write_u_all ? 1 : 0;
+  if (sh.wgb != 1 || sh.chunk != 1) {
+    if (sh.grid != 0) throw std::invalid_argument("work units of several blocks take one work-group per unit");
+    switch (sh.wgb) {
+      case 1: step_batch_units<S, 1>(t, sel, wu, sh, s, ev0, ev1); break;
+      case 2: step_batch_units<S, 2>(t, sel, wu, sh, s, ev0, ev1); break;
+      case 4: step_batch_units<S, 4>(t, sel, wu, sh, s, ev0, ev1); break;
+      default: throw std::invalid_argument("work-group size must be 256, 512 or 1024 threads");
+    }
+    return;
+  }
+  const int grid = sh.grid > 0 && sh.grid < t.nblk ? sh.grid : t.nblk;
   auto go = [&](auto kernel) { launch_maybe_timed(kernel, grid, s, ev0, ev1, t, sel, wu); };
   switch (sh.run * 2 + (sh.nt ? 1 : 0)) {
     case 0: go(kb_lbm_step<S, false, 0>); break;
@@ -729,6 +785,7 @@ void launch_lbm_step_batch(int storage, const BatchTables& t, int sel, bool writ
   // (members' distribution buffers are below 4 GiB: checked when they join)
   if (shape.run != 0 && shape.run != 4 && shape.run != 16 && shape.run != 64)
     throw std::invalid_argument("XCD run length must be 0, 4, 16 or 64");
+  if (shape.chunk < 1 || shape.chunk > 64) throw std::invalid_argument("blocks per round chain must be in [1, 64]");
   GLSM_STORAGE_DISPATCH(storage, (step_batch_impl<S>(t, sel, write_u_all, shape, s, ev0, ev1)));
   GLSM_LAUNCH_CHECK();
 }

@@ -63,15 +63,21 @@ class Batch {
 
   // Launch shape of the batched stream/collide kernel.
   struct Launch {
-    int nt;    // non-temporal stores
-    int run;   // XCD run length: 0 (one contiguous run per XCD), 4, 16 or 64
-    int grid;  // work-groups; 0 = one per 256-cell block, else persistent
+    int nt;         // non-temporal stores
+    int run;        // XCD run length: 0 (one contiguous run per XCD), 4, 16 or 64
+    int grid;       // work-groups; 0 = one per work unit, else persistent
+    int wgb = 1;    // 256-cell blocks stepped at once by a work-group: 1, 2, 4 (256-1024 threads)
+    int chunk = 1;  // rounds of blocks per work-group (cells per thread), 1..64
   };
   Launch launch() const;
   void setLaunch(const Launch& l);
   // Times every candidate shape on the members' current state (state is
   // not changed; each shape's time is the median of three trials of `reps`
   // launches) and keeps the fastest; returns it and its mean launch time.
+  // Candidates: NT stores x XCD run length x grid (one work-group per unit,
+  // or 2-16 per CU persistent) with one block per work-group, and work-group
+  // sizes of 512 / 1024 threads and 2-8 rounds of blocks per work-group
+  // (one work-group per unit).
   Launch autotune(int reps, double* best_ms);
 
   // Mean duration of the batched stream/collide kernel (start/stop events
@@ -115,7 +121,7 @@ class Batch {
   int pf_ = 0;  // which of a member's two distribution buffers holds its state
   int pc_ = 0;  // which of its two concentration buffers holds its state
   int nsub_ = 0;
-  Launch launch_{0, 0, 0};
+  Launch launch_{0, 0, 0, 1, 1};
   std::unique_ptr<glsm::BatchDevice> dev_;
   bool lists_fresh_ = false;        // IBM lists match the members' marker positions
   std::vector<uint64_t> seen_pos_;  // gels' position versions after the last step

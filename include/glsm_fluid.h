@@ -247,10 +247,12 @@ int glsm_batch_iteration(glsm_batch* b, int id);
 int glsm_batch_step(glsm_batch* b, int nsteps);
 int glsm_batch_use_graph(glsm_batch* b, int on);
 /* stream/collide launch shape: nt = non-temporal stores, run = XCD run length
-   (0, 4, 16 or 64), grid = work-groups (0 = one per 256-cell block, else
-   persistent) */
-int glsm_batch_set_launch(glsm_batch* b, int nt, int run, int grid);
-int glsm_batch_get_launch(glsm_batch* b, int* nt, int* run, int* grid);
+   (0, 4, 16 or 64), grid = work-groups (0 = one per work unit, else
+   persistent), wgb = 256-cell blocks stepped at once by a work-group (1, 2
+   or 4: 256, 512 or 1024 threads), chunk = rounds of consecutive blocks per
+   work-group (cells per thread, 1..64); a work unit is wgb * chunk blocks */
+int glsm_batch_set_launch(glsm_batch* b, int nt, int run, int grid, int wgb, int chunk);
+int glsm_batch_get_launch(glsm_batch* b, int* nt, int* run, int* grid, int* wgb, int* chunk);
 /* time every launch shape on the current state (unchanged) and keep the
    fastest; best_ms = its mean stream/collide launch time */
 int glsm_batch_autotune(glsm_batch* b, int reps, double* best_ms);

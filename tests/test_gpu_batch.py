@@ -92,7 +92,11 @@ def test_batch_members_equal_solo_runs(glsm, graph):
 def test_batch_fp16_and_every_launch_shape(glsm):
     G = glsm
     ref = [solo(G, s, 0, 6, "f16") for s in SPECS]
-    shapes = [(nt, run, grid) for nt in (0, 1) for run in (0, 4, 16, 64) for grid in (0, 3, 256)]
+    shapes = [(nt, run, grid, 1, 1) for nt in (0, 1) for run in (0, 4, 16, 64) for grid in (0, 3, 256)]
+    # larger work-groups and several rounds of blocks per work-group; units
+    # of up to 8 blocks straddle the members' block ranges
+    shapes += [(nt, run, 0, wgb, chunk) for nt in (0, 1) for run in (0, 4, 16)
+               for wgb, chunk in ((2, 1), (4, 1), (1, 3), (1, 8), (2, 2))]
     for shape in shapes:
         b = G.Batch("f16")
         insts = [build(G, s, "f16") for s in SPECS]
@@ -120,7 +124,7 @@ def test_batch_autotune_keeps_state_and_results(glsm):
         b.step(3)
         before = [state(inst) for inst in insts]
         shape, ms = b.autotune(reps=2)
-        assert ms > 0 and shape["run"] in (0, 4, 16, 64)
+        assert ms > 0 and shape["run"] in (0, 4, 16, 64) and shape["wgb"] in (1, 2, 4) and 1 <= shape["chunk"] <= 8
         for k, inst in enumerate(insts):
             assert_same(state(inst), before[k], f"autotune changed member {k}")
         b.step(5)
@@ -202,6 +206,8 @@ def test_batch_rejects_bad_members(glsm):
             b.remove(12345)
         with pytest.raises(G.GlsmError, match="run must be 0, 4, 16 or 64"):
             b.set_launch(0, 3, 0)
+        with pytest.raises(G.GlsmError, match="several blocks"):
+            b.set_launch(0, 4, 256, 2, 1)
     finally:
         b.close()
         close(inst)
@@ -294,3 +300,32 @@ def test_batch_sees_parameter_changes(glsm):
         b.close()
         close(solo_inst)
         close(member)
+
+
+def test_autotune_cache_reuses_the_pick(glsm, tmp_path):
+    """A pick cached under the batch's workload key is set again without
+    timing; another workload gets its own entry."""
+    import json
+
+    G = glsm
+    path = str(tmp_path / "shapes.json")
+    insts = [build(G, s) for s in SPECS]
+    try:
+        b = G.Batch("f32")
+        for inst in insts:
+            b.add(*inst)
+        shape, ms = b.autotune(reps=1, cache=path)
+        data = json.load(open(path))
+        assert list(data) == [b.workload_key()] and data[b.workload_key()]["launch"] == shape
+        b.set_launch(0, 64, 256, 1, 1)
+        again, ms2 = b.autotune(reps=1, cache=path)
+        assert again == shape and ms2 == ms
+        b.close()
+        b = G.Batch("f32")
+        b.add(*insts[0])
+        b.autotune(reps=1, cache=path)
+        assert len(json.load(open(path))) == 2
+        b.close()
+    finally:
+        for inst in insts:
+            close(inst)
