@@ -90,6 +90,7 @@ SIGNATURES = {
     "glsm_fluid_write": (_i, [_vp, _i, _vp, _sz]),
     "glsm_set_interp_mode": (_i, [_i]),
     "glsm_set_repulsion_cull": (_i, [_i]),
+    "glsm_set_lattice_xcd_run": (_i, [_i]),
     "glsm_batch_create": (_i, [_i, _i, C.POINTER(_vp)]),
     "glsm_batch_destroy": (_i, [_vp]),
     "glsm_batch_add": (_i, [_vp, _vp, _vp, _i, _i, C.c_char_p]),
@@ -169,6 +170,12 @@ def set_interp_mode(mode: int):
     1 = wavefront per marker, 2 = thread per marker, 3 = products staged in
     LDS (eight threads per marker)."""
     _check(load().glsm_set_interp_mode(int(mode)), "glsm_set_interp_mode")
+
+
+def set_lattice_xcd_run(run: int):
+    """XCD run length of the single-lattice stream/collide block order: -1
+    automatic (default), 0, 4, 16 or 64; results are identical."""
+    _check(load().glsm_set_lattice_xcd_run(int(run)), "glsm_set_lattice_xcd_run")
 
 
 def set_repulsion_cull(min_markers: int):

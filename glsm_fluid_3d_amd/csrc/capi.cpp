@@ -864,6 +864,12 @@ int glsm_device_memory(size_t* free_bytes, size_t* total_bytes) {
   });
 }
 
+int glsm_set_lattice_xcd_run(int run) {
+  return guarded([&] {
+    glsm::set_lattice_xcd_run(run);
+    return GLSM_OK;
+  });
+}
 int glsm_set_interp_mode(int mode) {
   return guarded([&] {
     glsm::set_interp_mode(mode);

@@ -299,11 +299,20 @@ void Fluid::subStep(float beta_eff, bool write_u_all) {
   std::swap(d_f, d_fnext);
 }
 
+// The state is converted on the device: exported as FP32 pre-collision
+// populations into a scratch buffer, then imported in the new storage (the
+// same values a host round trip would carry, without the PCIe copies).
 void Fluid::setStorage(int storage, int scale_exp) {
   if (storage < F32_FULL || storage > BF16_DEV) throw std::invalid_argument("setStorage: unknown storage mode");
   if (scale_exp < 0 || scale_exp > 30) throw std::invalid_argument("setStorage: scale_exp must be in [0, 30]");
-  std::vector<float> host(19ull * N);
-  readPopulations(host.data());
+  if (storage == storage_ && (storage != F16_DEV || scale_exp == scale_exp_)) {
+    scale_exp_ = scale_exp;
+    return;
+  }
+  float* tmp = glsm::dalloc<float>(Nd);
+  glsm::launch_lbm_export(storage_, true, geom(*this), lbm_params(*h_fp, storage_, scale_exp_), d_f, tmp, plane,
+                          plane, fluid_stream);
+  GLSM_HIP(hipStreamSynchronize(fluid_stream));
   glsm::dfree(d_f);
   glsm::dfree(d_fnext);
   d_f = d_fnext = nullptr;
@@ -311,7 +320,10 @@ void Fluid::setStorage(int storage, int scale_exp) {
   scale_exp_ = scale_exp;
   d_f = glsm::dalloc<char>(Nd * glsm::storage_bytes(storage_));
   d_fnext = glsm::dalloc<char>(Nd * glsm::storage_bytes(storage_));
-  writePopulations(host.data());
+  glsm::launch_lbm_import(storage_, geom(*this), lbm_params(*h_fp, storage_, scale_exp_), tmp, d_f, plane, plane,
+                          fluid_stream);
+  GLSM_HIP(hipStreamSynchronize(fluid_stream));
+  glsm::dfree(tmp);
 }
 
 // fluid.cu:251-261

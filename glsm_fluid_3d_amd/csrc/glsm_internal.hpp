@@ -218,6 +218,10 @@ struct LbmParams {
 void launch_lbm_step(int storage, const LatticeGeom& g, const LbmParams& p, const void* fin, void* fout,
                      size_t plane, float3* u, bool write_u_all, const SpreadLists& sl, const float3* Fl,
                      hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+// XCD run length of the single-lattice stream/collide block order
+// (-1 = automatic: 4 when the state streams from HBM, else 0); a
+// measurement switch, process-wide
+void set_lattice_xcd_run(int run);
 void launch_lbm_init(int storage, const LatticeGeom& g, const LbmParams& p, void* f, size_t plane,
                      float3* u, float* c1, float* c2, hipStream_t s);
 // stored state -> FP32 full populations (pull = pre-collision of the next

@@ -30,6 +30,7 @@
 //              keep ~1e-7 relative precision on it;
 //   F16_DEV  : d * 2^s in binary16 (s = scale exponent, default 20);
 //   BF16_DEV : d in bfloat16 (round to nearest even).
+#include <atomic>
 #include <cstddef>
 #include <map>
 #include <mutex>
@@ -387,14 +388,15 @@ __device__ __forceinline__ void lbm_block(const LatticeGeom& g, const LbmParams&
   }
 }
 
-// 8 waves per SIMD (<= 64 VGPRs): keeps enough loads in flight per CU
-template <int S, bool NT, bool B32>
+// 8 waves per SIMD (<= 64 VGPRs): keeps enough loads in flight per CU.
+// RUN: the XCD run length of the block order (xcd_block).
+template <int S, bool NT, bool B32, int RUN>
 __global__ void __launch_bounds__(kThreads, 8) k_lbm_step(LatticeGeom g, LbmParams p,
                                                        const typename Store<S>::T* __restrict__ fin,
                                                        typename Store<S>::T* __restrict__ fout, unsigned plane,
                                                        float3* __restrict__ u, int write_u_all, SpreadLists sl,
                                                        const float3* __restrict__ Fl, int const_force) {
-  const unsigned b = xcd_block<NT ? 4 : 0>(blockIdx.x, gridDim.x);
+  const unsigned b = xcd_block<RUN>(blockIdx.x, gridDim.x);
   lbm_block<S, NT, B32>(g, p, fin, fout, plane, u, write_u_all, sl, Fl, const_force, b, threadIdx.x);
 }
 
@@ -672,6 +674,8 @@ void launch_maybe_timed(K kernel, int grid, hipStream_t s, hipEvent_t ev0, hipEv
 // a distribution buffer whose every byte offset fits in 32 bits
 bool fits_b32(size_t plane, size_t elem) { return 19ull * plane * elem < (1ull << 32); }
 
+std::atomic<int> g_lattice_run{-1};  // forced XCD run length of the single-lattice kernel, -1 = automatic
+
 template <int S>
 void step_impl(const LatticeGeom& g, const LbmParams& p, const void* fin, void* fout, size_t plane, float3* u,
                bool write_u_all, const SpreadLists& sl, const float3* Fl, hipStream_t s, hipEvent_t ev0,
@@ -689,15 +693,37 @@ void step_impl(const LatticeGeom& g, const LbmParams& p, const void* fin, void* 
     launch_maybe_timed(kernel, blocks_for(g.N), s, ev0, ev1, g, p, in, out, (unsigned)plane, u, wu, sl, Fl,
                        const_force);
   };
-  if (nt)
-    b32 ? go(k_lbm_step<S, true, true>) : go(k_lbm_step<S, true, false>);
-  else
-    b32 ? go(k_lbm_step<S, false, true>) : go(k_lbm_step<S, false, false>);
+  // XCD run length: 4-block runs once the state streams from HBM, one
+  // contiguous run per XCD while it stays in the Infinity Cache, unless
+  // set_lattice_xcd_run() forced another
+  const int forced = g_lattice_run.load(std::memory_order_relaxed);
+  const int run = forced >= 0 ? forced : (nt ? 4 : 0);
+  if (!b32) {
+    nt ? go(k_lbm_step<S, true, false, 4>) : go(k_lbm_step<S, false, false, 0>);
+    return;
+  }
+  switch (run * 2 + (nt ? 1 : 0)) {
+    case 0: go(k_lbm_step<S, false, true, 0>); break;
+    case 1: go(k_lbm_step<S, true, true, 0>); break;
+    case 8: go(k_lbm_step<S, false, true, 4>); break;
+    case 9: go(k_lbm_step<S, true, true, 4>); break;
+    case 32: go(k_lbm_step<S, false, true, 16>); break;
+    case 33: go(k_lbm_step<S, true, true, 16>); break;
+    case 128: go(k_lbm_step<S, false, true, 64>); break;
+    case 129: go(k_lbm_step<S, true, true, 64>); break;
+    default: throw std::invalid_argument("XCD run length must be 0, 4, 16 or 64");
+  }
 }
 
 }  // namespace
 
 size_t storage_bytes(int storage) { return storage == F16_DEV || storage == BF16_DEV ? 2 : 4; }
+
+void set_lattice_xcd_run(int run) {
+  if (run != -1 && run != 0 && run != 4 && run != 16 && run != 64)
+    throw std::invalid_argument("XCD run length must be -1 (automatic), 0, 4, 16 or 64");
+  g_lattice_run = run;
+}
 
 void launch_copy16(const void* a, void* b, size_t bytes, hipStream_t s) {
   const size_t n4 = bytes / 16;

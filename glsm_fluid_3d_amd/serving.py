@@ -108,7 +108,7 @@ class Server:
 
     def __init__(self, mod, storage="f16", hbm_budget=None, max_active_cells=16_000_000, max_active_jobs=256,
                  out_root=None, keep_output=True, writers=4, max_chunk=50, autotune=True, on_complete=None,
-                 health_every=100):
+                 health_every=500):
         self.G = mod
         self.storage = storage
         free, total = mod.device_memory()
@@ -135,6 +135,9 @@ class Server:
         self.cell_updates = 0
         self.steps = 0
         self.tuned = False
+        # wall seconds spent admitting (building jobs), stepping (until the
+        # device is idle) and retiring / checking jobs
+        self.host_s = {"admit": 0.0, "step": 0.0, "retire_and_checks": 0.0}
 
     # ---- admission ---------------------------------------------------------
     def submit(self, job: W.Job):
@@ -248,7 +251,9 @@ class Server:
                 break
             while pending and pending[0].arrival <= now:
                 self.submit(pending.popleft())
+            t_a = time.perf_counter()
             self._admit(now)
+            self.host_s["admit"] += time.perf_counter() - t_a
             if not self.active:
                 if not pending and not self.queue:
                     break
@@ -259,14 +264,18 @@ class Server:
                 continue
             left = min(r.job.horizon - self.batch.iteration(r.member) for r in self.active.values())
             n = max(1, min(self.max_chunk, left))
+            t_s = time.perf_counter()
             self.batch.step(n)
             self.G.synchronize()
+            self.host_s["step"] += time.perf_counter() - t_s
             self.steps += n
             self.cell_updates += self.active_cells * NSUB * n
             if clock is not None:
                 clock.advance(n)
             check = self.health_every and (self.steps // self.health_every) != ((self.steps - n) // self.health_every)
+            t_r = time.perf_counter()
             self._retire(now_fn(), check_all=bool(check))
+            self.host_s["retire_and_checks"] += time.perf_counter() - t_r
             if progress_every and now >= next_report:
                 next_report += progress_every
                 log(f"[serve {now:7.1f} s] active {len(self.active)} queued {len(self.queue)} "
@@ -290,6 +299,7 @@ class Server:
             "mlups": self.cell_updates / max(self.elapsed, 1e-9) / 1e6 if hasattr(self, "elapsed") else None,
             "elapsed_s": getattr(self, "elapsed", None), "batch_steps": self.steps,
             "hbm_budget_GB": self.hbm_budget / 1e9,
+            "wall_s_by_phase": {k: round(v, 3) for k, v in self.host_s.items()},
         }
         if self.output:
             out["output"] = self.output.status()

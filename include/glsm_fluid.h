@@ -219,6 +219,14 @@ int glsm_copy_bandwidth(size_t bytes, int reps, double* gbps);
    identical.  Process-wide; launches issued after the call use it. */
 int glsm_set_interp_mode(int mode);
 
+/* Extension (measurement): XCD run length of the single-lattice
+   stream/collide kernel's block order -- consecutive 256-cell blocks dealt
+   to one XCD (whose L2 then serves the cache lines neighbouring blocks
+   share): -1 = automatic (4 when the state streams from HBM, 0 = one
+   contiguous run per XCD while it fits the Infinity Cache), else 0, 4, 16
+   or 64.  Results are identical.  Process-wide. */
+int glsm_set_lattice_xcd_run(int run);
+
 /* Extension (measurement and tests): from `min_markers` markers on (default
    4096; 0 = never), Coupler::applyGelRepulsion skips, for each marker, the
    gels whose markers' bounding box lies beyond the cut-off.  The sum is
