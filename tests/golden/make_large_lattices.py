@@ -12,6 +12,8 @@ workloads, for the GPU parity tests of the plain and precision forms:
   walled201_four_gels_full.npz 201^3 nodes (box 100), walls, four 8^3-element
                                gels at seeded non-overlapping centres, ramp 20;
                                50 and 100 steps
+  walled201_four_gels_ref.npz  the same scene with the reference's IBM ramp;
+                               50 and 100 steps
 
 Each checkpoint holds, from the oracle's FP64 state:
   * u, c and rho - 1 (rho = sum of the next sub-step's pre-collision
@@ -28,10 +30,11 @@ Full fields would be 12-100 MB per checkpoint; the sample keeps each fixture
 near 1 MB while still giving a relative L2 over ~26k cells, a third of them
 in the planes where the gels force the fluid.
 
-Run from the repository root (the 201^3 case takes about an hour on 8 cores):
+Run from the repository root (each 201^3 case takes about half an hour on 8 cores):
     python tests/golden/make_large_lattices.py walled101_two_gels_ref
     python tests/golden/make_large_lattices.py walled101_two_gels_full
     python tests/golden/make_large_lattices.py walled201_four_gels_full
+    python tests/golden/make_large_lattices.py walled201_four_gels_ref
 """
 import os
 import sys
@@ -54,6 +57,7 @@ CASES = {
     "walled101_two_gels_ref": dict(scene="two", ramp=None, checkpoints=(100, 200), window=50),
     "walled101_two_gels_full": dict(scene="two", ramp=20.0, checkpoints=(100, 200), window=50),
     "walled201_four_gels_full": dict(scene="four", ramp=20.0, checkpoints=(50, 100), window=24),
+    "walled201_four_gels_ref": dict(scene="four", ramp=None, checkpoints=(50, 100), window=24),
 }
 
 
