@@ -17,7 +17,11 @@ Jobs (lattice, gels, rate constants, horizon) arrive over time.  The server
     fields; an FP16 deviation past the storage's range is stored as Inf and
     counts) and stops a failed job at once, listing it as failed.
 The loop runs on the host between chunks of steps; a chunk never passes the
-nearest horizon, so completions are exact to the step.
+nearest horizon, so completions are exact to the step.  While a chunk runs
+on the device the host builds the objects of the queued jobs that already
+fit (their allocations, setup kernels and uploads queue behind the chunk),
+so admission after the chunk only joins them to the batch; which jobs are
+admitted, and when, is the same as building them at admission.
 """
 from __future__ import annotations
 
@@ -126,6 +130,7 @@ class Server:
         if self.output:
             self.batch.set_output(self.output)
         self.queue = deque()
+        self.ready = deque()  # (record, objects): the queue's head, built during the last chunk
         self.active = {}
         self.done = []
         self.rejected = []
@@ -135,9 +140,11 @@ class Server:
         self.cell_updates = 0
         self.steps = 0
         self.tuned = False
-        # wall seconds spent admitting (building jobs), stepping (until the
-        # device is idle) and retiring / checking jobs
-        self.host_s = {"admit": 0.0, "step": 0.0, "retire_and_checks": 0.0}
+        # wall seconds spent admitting (joining jobs to the batch, building
+        # those not built yet), stepping (until the device is idle; includes
+        # the jobs built meanwhile, also counted apart) and retiring /
+        # checking jobs
+        self.host_s = {"admit": 0.0, "step": 0.0, "build_during_step": 0.0, "retire_and_checks": 0.0}
 
     # ---- admission ---------------------------------------------------------
     def submit(self, job: W.Job):
@@ -152,18 +159,49 @@ class Server:
         self.queue.append(rec)
         return True
 
-    def _fits(self, rec: JobRecord) -> bool:
-        if len(self.active) >= self.max_active_jobs:
+    def _fits(self, rec: JobRecord, jobs=0, nbytes=0, cells=0) -> bool:
+        """Whether `rec` fits beside the active jobs and `jobs` more holding
+        `nbytes` and `cells`."""
+        if len(self.active) + jobs >= self.max_active_jobs:
             return False
-        if self.used_bytes + rec.footprint > self.hbm_budget:
+        if self.used_bytes + nbytes + rec.footprint > self.hbm_budget:
             return False
-        return not self.active or self.active_cells + rec.job.cells <= self.max_active_cells
+        busy = self.active_cells + cells
+        return not (self.active or jobs) or busy + rec.job.cells <= self.max_active_cells
+
+    def _prebuild(self):
+        """Build the objects of the queued jobs that fit now, beside those
+        already built, in queue order (called while a chunk runs)."""
+        jobs = len(self.ready)
+        nbytes = sum(r.footprint for r, _ in self.ready)
+        cells = sum(r.job.cells for r, _ in self.ready)
+        while self.queue and self._fits(self.queue[0], jobs, nbytes, cells):
+            rec = self.queue.popleft()
+            self.ready.append((rec, W.build(self.G, rec.job, self.storage)))
+            jobs += 1
+            nbytes += rec.footprint
+            cells += rec.job.cells
+
+    def _next_admissible(self):
+        """The queue's head (built or not) if it fits, else None."""
+        if self.ready:
+            return self.ready[0] if self._fits(self.ready[0][0]) else None
+        if self.queue and self._fits(self.queue[0]):
+            return self.queue[0], None
+        return None
 
     def _admit(self, now: float) -> int:
         n = 0
-        while self.queue and self._fits(self.queue[0]):
-            rec = self.queue.popleft()
-            objs = W.build(self.G, rec.job, self.storage)
+        while True:
+            head = self._next_admissible()
+            if head is None:
+                break
+            rec, objs = head
+            if objs is None:
+                self.queue.popleft()
+                objs = W.build(self.G, rec.job, self.storage)
+            else:
+                self.ready.popleft()
             if self.out_root:
                 rec.out_dir = os.path.join(self.out_root, f"job{rec.job.job_id}")
                 os.makedirs(rec.out_dir, exist_ok=True)
@@ -255,7 +293,7 @@ class Server:
             self._admit(now)
             self.host_s["admit"] += time.perf_counter() - t_a
             if not self.active:
-                if not pending and not self.queue:
+                if not pending and not self.queue and not self.ready:
                     break
                 if clock is None:
                     time.sleep(idle_sleep)
@@ -266,6 +304,9 @@ class Server:
             n = max(1, min(self.max_chunk, left))
             t_s = time.perf_counter()
             self.batch.step(n)
+            t_b = time.perf_counter()
+            self._prebuild()
+            self.host_s["build_during_step"] += time.perf_counter() - t_b
             self.G.synchronize()
             self.host_s["step"] += time.perf_counter() - t_s
             self.steps += n
@@ -291,7 +332,7 @@ class Server:
         wait = np.array([r.admitted - r.job.arrival for r in self.done]) if self.done else np.zeros(0)
         pct = lambda a, q: float(np.percentile(a, q)) if a.size else None
         out = {
-            "completed": len(self.done), "active": len(self.active), "queued": len(self.queue),
+            "completed": len(self.done), "active": len(self.active), "queued": len(self.queue) + len(self.ready),
             "rejected": [r.job.job_id for r in self.rejected],
             "failed": {r.job.job_id: r.extra["failure"] for r in self.failed},
             "latency_p50_s": pct(lat, 50), "latency_p99_s": pct(lat, 99),
@@ -315,6 +356,10 @@ class Server:
             for o in gels + [f, c]:
                 o.close()
         self.active.clear()
+        while self.ready:
+            gels, c, f = self.ready.popleft()[1]
+            for o in gels + [f, c]:
+                o.close()
         if self.output:
             self.output.close()
         self.batch.close()
