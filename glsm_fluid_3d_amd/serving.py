@@ -160,21 +160,23 @@ class Server:
         return True
 
     def _fits(self, rec: JobRecord, jobs=0, nbytes=0, cells=0) -> bool:
-        """Whether `rec` fits beside the active jobs and `jobs` more holding
-        `nbytes` and `cells`."""
-        if len(self.active) + jobs >= self.max_active_jobs:
+        """Whether `rec` fits beside the active jobs changed by `jobs` jobs
+        holding `nbytes` and `cells` (negative: jobs about to leave)."""
+        n_active = len(self.active) + jobs
+        if n_active >= self.max_active_jobs:
             return False
         if self.used_bytes + nbytes + rec.footprint > self.hbm_budget:
             return False
-        busy = self.active_cells + cells
-        return not (self.active or jobs) or busy + rec.job.cells <= self.max_active_cells
+        return n_active == 0 or self.active_cells + cells + rec.job.cells <= self.max_active_cells
 
-    def _prebuild(self):
-        """Build the objects of the queued jobs that fit now, beside those
-        already built, in queue order (called while a chunk runs)."""
-        jobs = len(self.ready)
-        nbytes = sum(r.footprint for r, _ in self.ready)
-        cells = sum(r.job.cells for r, _ in self.ready)
+    def _prebuild(self, leaving=()):
+        """Build the objects of the queued jobs that will fit once the jobs
+        in `leaving` (those reaching their horizon with the running chunk)
+        are retired, beside those already built, in queue order (called while
+        the chunk runs)."""
+        jobs = len(self.ready) - len(leaving)
+        nbytes = sum(r.footprint for r, _ in self.ready) - sum(r.footprint for r in leaving)
+        cells = sum(r.job.cells for r, _ in self.ready) - sum(r.job.cells for r in leaving)
         while self.queue and self._fits(self.queue[0], jobs, nbytes, cells):
             rec = self.queue.popleft()
             self.ready.append((rec, W.build(self.G, rec.job, self.storage)))
@@ -305,7 +307,8 @@ class Server:
             t_s = time.perf_counter()
             self.batch.step(n)
             t_b = time.perf_counter()
-            self._prebuild()
+            # (iteration counters advance when the steps are enqueued)
+            self._prebuild([r for r in self.active.values() if self.batch.iteration(r.member) >= r.job.horizon])
             self.host_s["build_during_step"] += time.perf_counter() - t_b
             self.G.synchronize()
             self.host_s["step"] += time.perf_counter() - t_s
