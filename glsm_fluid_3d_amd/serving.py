@@ -173,9 +173,10 @@ class Server:
         """Build the objects of the queued jobs that will fit once the jobs
         in `leaving` (those reaching their horizon with the running chunk)
         are retired, beside those already built, in queue order (called while
-        the chunk runs)."""
+        the chunk runs).  The leaving jobs' device memory still counts: it is
+        freed only after the chunk, and the new objects allocate now."""
         jobs = len(self.ready) - len(leaving)
-        nbytes = sum(r.footprint for r, _ in self.ready) - sum(r.footprint for r in leaving)
+        nbytes = sum(r.footprint for r, _ in self.ready)
         cells = sum(r.job.cells for r, _ in self.ready) - sum(r.job.cells for r in leaving)
         while self.queue and self._fits(self.queue[0], jobs, nbytes, cells):
             rec = self.queue.popleft()
