@@ -5,14 +5,17 @@ fingerprints in tests/golden/ (written by tests/golden/make_large_lattices.py):
   * walled101_two_gels_ref / _full: 101^3 nodes with walls, two 6^3 gels
     placed as in sim.cpp, the reference's IBM ramp (20000 steps) and a
     20-step ramp (full IBM stiffness from step 20), 200 steps;
-  * walled201_four_gels_full: 201^3 nodes with walls, four 8^3 gels at
-    seeded non-overlapping centres, 20-step ramp, 100 steps.
+  * walled201_four_gels_ref / _full: 201^3 nodes with walls, four 8^3
+    gels at seeded non-overlapping centres, the reference's IBM ramp and a
+    20-step ramp, 100 steps.
 Per storage (FP32 deviations, FP16 and BF16 deviations with FP64 moments)
 the relative L2 errors of u, c and rho - 1 over the fixture's ~26k sampled
 cells, the largest node-position error and the FP64 mass / momentum sums
 are printed; tolerances, at every checkpoint:
-  FP32: u <= 1e-5, c <= 3e-6, rho - 1 <= 1e-5, rn <= 5e-7 gel units
-        (c reaches 1.1e-6 on the 201^3 lattice by step 50);
+  FP32: u <= 1e-5 (2e-5 on the 201^3 lattice with the reference's ramp,
+        where the velocity signal is 1e-6 at step 50), c <= 3e-6, rho - 1 <=
+        1e-5, rn <= 5e-7 gel units (c reaches 1.1e-6 on the 201^3 lattice by
+        step 50);
   FP16: u <= 1e-3 (the north_star's velocity bound) with the reference's
         IBM ramp; 2e-3 at full IBM stiffness from step 20 (measured 1.1e-3 at
         101^3, 1.5e-3 at 201^3), c <= 1e-4, rho - 1 <= 2e-3;
@@ -59,6 +62,11 @@ TOL = {
     "bf16": {"u": 1e-2, "c": 1e-3, "rho": 2e-2},
 }
 TOL_FULL_IBM = {"f16": {"u": 2e-3}, "bf16": {"u": 2e-2}}
+# 201^3 with the reference's ramp: at step 50 the IBM stiffness is 0.25% of
+# full, rms |u| over the sample is 1.2e-6 against rms (rho - 1) 1.5e-5, and
+# FP32's accumulated rounding of the stored deviations (~2e-11 absolute
+# after 1250 sub-steps) is 1.5e-5 of the velocity signal
+TOL_CASE = {("walled201_four_gels_ref", "f32"): {"u": 2e-5}}
 
 
 def say(*a):
@@ -81,7 +89,8 @@ def golden_errors(gold, n, gels, f):
 
 
 @pytest.mark.parametrize("storage", ["f32", "f16", "bf16"])
-@pytest.mark.parametrize("case", ["walled101_two_gels_ref", "walled101_two_gels_full", "walled201_four_gels_full"])
+@pytest.mark.parametrize("case", ["walled101_two_gels_ref", "walled101_two_gels_full", "walled201_four_gels_ref",
+                                  "walled201_four_gels_full"])
 def test_large_lattice_vs_fp64_fingerprints(glsm, case, storage):
     import make_large_lattices as M
 
@@ -95,6 +104,7 @@ def test_large_lattice_vs_fp64_fingerprints(glsm, case, storage):
         tol = dict(TOL[storage])
         if M.CASES[case]["ramp"] is not None:
             tol.update(TOL_FULL_IBM.get(storage, {}))
+        tol.update(TOL_CASE.get((case, storage), {}))
         for it in range(cps[-1]):
             glsm.system_step(gels, c, f, it)
             if it + 1 in cps:
