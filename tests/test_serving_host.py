@@ -175,3 +175,16 @@ def test_step_clock_and_horizons_exact():
         # is at least its horizon in steps
         assert completed - admitted >= 0.05 * steps - 1e-9
     assert np.isfinite([r[2] for r in rec]).all()
+
+
+def test_close_releases_active_and_built_ahead_jobs():
+    jobs = _jobs(40)
+    budget = 40 * S.job_footprint(jobs[0], "f16", False)
+    _Obj.made.clear()
+    srv = S.Server(_Lib(budget / 0.8), storage="f16", autotune=False, max_active_cells=4_000_000)
+    srv.run(jobs, duration=3.0, clock=S.StepClock(dt=0.05))
+    assert srv.active and srv.queue
+    srv._prebuild(list(srv.active.values()))  # as if a chunk retiring every active job were running
+    assert srv.ready
+    srv.close()
+    assert _Obj.made and all(o.closed for o in _Obj.made)
