@@ -22,6 +22,10 @@ iterations; the same sequence through the C ABI must print the same lines
 and write the same .dat files, and for sim_gelonly the printed tip
 deflection and vm maximum must match the FP64 oracle.
 
+sim_peristaltic.cpp (two 40x40x6 slabs in an 81x81x61 channel with
+GEL_SIZE_X=40, pulse at iteration 500) runs the coupled step with two gels
+for 1000 iterations: its status lines and files are compared the same way.
+
 The sim_single scene is then driven through the C ABI from Python with the
 driver's call sequence -- per-object methods before iteration 500,
 glsm_system_step (sim.cpp's loop body, the same calls) from there on -- and
@@ -264,3 +268,69 @@ def test_sim_gelonly_tube_driver_prints_and_writes_the_same(glsm, tmp_path):
     same_files(run_dir / "data_gelonly_tube", mir,
                {f"gel1{k}{t}.dat" for k in ("rn", "Fn", "Veln", "rm", "um", "vm", "wm") for t in (0, 1)},
                skip={"gel1Veln0.dat", "gel1um0.dat", "gel1vm0.dat"})
+
+
+# ------------------------------------------------------------ two slabs
+def peristaltic_sequence(G, runstep, gel_lx=40, wave_delay=500):
+    """sim_peristaltic.cpp:56-140 through the C ABI: status lines and files."""
+    size = (gel_lx, 40, 6)
+    top = G.Gel(size, (gel_lx / 2.0, 20.0, 26.0), gel_type=1, gel_id=1)
+    bot = G.Gel(size, (gel_lx / 2.0, 20.0, 4.0), gel_type=1, gel_id=2)
+    top.reset_to_quiescent()
+    bot.reset_to_quiescent()
+    gels = [top, bot]
+    c = G.Coupler(gels)
+    f = G.Fluid((gel_lx, 40, 30), c)
+    h = gel_lx / (f.grid[0] - 1)
+    lines = []
+    for it in range(runstep + 1):
+        if it == wave_delay:
+            top.fire_excitation_pulse()
+            bot.fire_excitation_pulse()
+            lines.append(f"iter={it}: BZ excitation pulse fired on both slabs")
+        if it < wave_delay:
+            for g in gels:
+                g.record_center(it)
+            c.pack_from_gels()
+            f.step_velocity(it)
+            f.step_concentration(it)
+            c.apply_gel_repulsion()
+            c.scatter_to_gels()
+        else:
+            G.system_step(gels, c, f, it)  # the same calls, in the driver's order
+        if it % 1000 == 0:
+            vg = np.float32(0.0)
+            for g in gels:
+                v = g.read("Veln")
+                vg = max(vg, np.sqrt(v[:, 0] * v[:, 0] + v[:, 1] * v[:, 1] + v[:, 2] * v[:, 2]).astype(np.float32).max())
+            lines.append("iter=%6d  Vg_max=%.5f  lbm2gel=%.2f" % (it, float(vg), np.float32(f.nsub) / np.float32(h)))
+        for g in gels:
+            g.write_files(it)
+        f.write_files(it)
+    for o in gels + [f, c]:
+        o.close()
+    return lines
+
+
+def test_sim_peristaltic_driver_prints_and_writes_the_same(glsm, tmp_path):
+    runstep = 1000
+    run_dir = tmp_path / "driver"
+    run_dir.mkdir()
+    out = run_driver("sim_peristaltic", run_dir, {"RUNSTEP": str(runstep), "GEL_SIZE_X": "40"})
+    printed = [ln for ln in out.splitlines() if ln.startswith("iter=")]
+    mir = tmp_path / "mirror"
+    mir.mkdir()
+    cwd = os.getcwd()
+    os.chdir(mir)
+    try:
+        lines = peristaltic_sequence(glsm, runstep)
+    finally:
+        os.chdir(cwd)
+    print("\n".join(lines), flush=True)
+    assert printed == lines
+    # as in the tube driver, the host copies of m_dVeln at iteration 1000
+    # may overlap the writer threads of iteration 0 (gel.cu:543-553)
+    gel_files = {f"gel{k}{n}{t}.dat" for k in (1, 2) for n in ("rn", "Fn", "Veln", "rm", "um", "vm", "wm")
+                 for t in (0, 1)}
+    same_files(run_dir / "data", mir, gel_files | {"Velb0.dat", "Conc0.dat"},
+               skip={"gel1Veln0.dat", "gel2Veln0.dat"})
