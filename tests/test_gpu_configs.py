@@ -16,7 +16,7 @@ are printed; tolerances, at every checkpoint:
         where the velocity signal is 1e-6 at step 50), c <= 3e-6, rho - 1 <=
         1e-5, rn <= 5e-7 gel units (c reaches 1.1e-6 on the 201^3 lattice by
         step 50);
-  FP16: u <= 1e-3 (the north_star's velocity bound) with the reference's
+  FP16: u <= 1e-3 (the precision form's velocity target) with the reference's
         IBM ramp; 2e-3 at full IBM stiffness from step 20 (measured 1.1e-3 at
         101^3, 1.5e-3 at 201^3), c <= 1e-4, rho - 1 <= 2e-3;
   BF16: u <= 1e-2 (2e-2 at full stiffness: 1.1e-2 at 101^3, 1.5e-2 at
@@ -53,7 +53,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(HERE, "golden"))
 
 # relative L2 bounds at every checkpoint; with the reference's IBM ramp FP16
-# keeps the north_star's 1e-3 on u; at full IBM stiffness from step 20 (a
+# keeps the 1e-3 velocity target on u; at full IBM stiffness from step 20 (a
 # far stronger forcing: rho - 1 reaches 2e-2 near the gels) FP16 reaches
 # 1.1e-3 at 101^3 and 1.5e-3 at 201^3
 TOL = {
