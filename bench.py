@@ -302,8 +302,8 @@ def main():
     if args.workload != "all":
         args.no_cpu_baseline = True
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for {args.gpus}); the time step runs on a single "
-              "device and multi-GPU runs are not part of this build.", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for {args.gpus}): the time step runs on a single "
+              "device, and no multi-GPU path is implemented.", file=sys.stderr)
         sys.exit(2)
 
     import glsm_fluid_3d_amd as G

@@ -1,8 +1,8 @@
 """ctypes binding of the CPU restatement (oracle/liboracle.so).
 
 TEST INFRASTRUCTURE ONLY.  Only tests/, ``__graft_entry__.smoke()`` and
-bench.py's ``cpu_baseline`` leg import this module, as the checker or as the
-timed CPU baseline.  The product (``glsm_fluid_3d_amd``) never imports it.
+bench.py's ``cpu_baseline`` leg import this module, as the oracle the
+library's results are compared with or as the timed CPU baseline.  The product (``glsm_fluid_3d_amd``) never imports it.
 
 The classes mirror the reference's ``Gel`` / ``Coupler`` / ``Fluid``
 (src/gel.h, src/coupling.h, src/fluid.h) and use the field numbering of

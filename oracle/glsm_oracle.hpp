@@ -3,7 +3,8 @@
 // TEST INFRASTRUCTURE ONLY.  Nothing in the shipped library links, calls or
 // includes this file.  Only tests/, __graft_entry__.smoke() and bench.py's
 // cpu_baseline leg load it (through oracle/liboracle.so), and only as the
-// checker or as the timed CPU baseline.
+// oracle the library's results are compared with or as the timed CPU
+// baseline.
 //
 // What it restates (reference file:line for every routine is given at the
 // routine):
