@@ -56,3 +56,24 @@ def test_large_lattice_64bit_addressing_matches_tiled_small(glsm):
     finally:
         for o in (fs, fb, cs, cb, gs, gb):
             o.close()
+
+
+def test_batch_rejects_a_member_over_4gib(glsm):
+    """The batched kernels address a member's buffers by 32-bit byte
+    offsets: Batch.add refuses a lattice whose FP32 distributions reach
+    4 GiB, and the batch stays usable."""
+    G = glsm
+    g, c, f = make(G, tuple(TILE * r for r in REPS))
+    gs, cs, fs = make(G, (TILE, TILE, TILE))
+    b = G.Batch("f32")
+    try:
+        assert 19 * f.N * 4 >= 1 << 32
+        with pytest.raises(G.GlsmError, match="too large for a batch member"):
+            b.add([g], c, f)
+        mid = b.add([gs], cs, fs)
+        b.step(1)
+        assert b.iteration(mid) == 1
+    finally:
+        b.close()
+        for o in (f, fs, c, cs, g, gs):
+            o.close()
