@@ -173,8 +173,26 @@ def test_pack_repulsion_scatter(glsm, oracle, cull):
 @pytest.mark.parametrize("per", [0, 1])
 @pytest.mark.parametrize("storage", ["f32full", "f32", "f16", "bf16"])
 def test_lbm_substep(glsm, oracle, storage, per):
-    say("sub-step", storage, per)
-    g, og, og64 = (x.Gel((6, 6, 6), (10, 10, 10)) for x in (glsm, oracle, oracle))
+    substep_case(glsm, oracle, storage, per, (10, 10, 10))
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16"])
+@pytest.mark.parametrize("per,pos", [(0, (3.4, 10.0, 16.6)), (1, (1.0, 10.0, 19.5))],
+                         ids=["clipped-at-walls", "wrapped-periodic"])
+def test_lbm_substep_stencils_at_the_boundary(glsm, oracle, storage, per, pos):
+    """Marker stencils cut by the walls (clipped to the lattice, weights
+    renormalised by the clipped sum) and, in a periodic box, marker stencils
+    that wrap across the faces: the same bits as the FP32 restatement for Ul
+    and Fdrag, the sub-step within the storage's tolerance of FP64."""
+    g, grid = substep_case(glsm, oracle, storage, per, pos)
+    lo, hi = np.floor(g).min(axis=0) - 1, np.floor(g).max(axis=0) + 2  # stencil extent per axis
+    outside = (lo < 0) | (hi > np.array(grid) - 1)
+    assert outside[0] and outside[2], (lo, hi, grid)  # some stencils leave the lattice in x and z
+
+
+def substep_case(glsm, oracle, storage, per, pos):
+    say("sub-step", storage, per, pos)
+    g, og, og64 = (x.Gel((6, 6, 6), pos) for x in (glsm, oracle, oracle))
     c, oc, oc64 = glsm.Coupler([g]), oracle.Coupler([og], 32), oracle.Coupler([og64], 64)
     f, of, of64 = glsm.Fluid((20, 20, 20), c), oracle.Fluid((20, 20, 20), oc), oracle.Fluid((20, 20, 20), oc64)
     for fl in (f, of, of64):
@@ -210,6 +228,7 @@ def test_lbm_substep(glsm, oracle, storage, per):
     tol_u = {"f32full": 1e-6, "f32": 2e-7, "f16": 2e-4, "bf16": 2e-3}[storage]
     assert rel_l2(fg, ref_f) < tol_f
     assert rel_l2(ug, ref_u) < tol_u
+    return c.read("lag").astype(np.float64) / 0.5, f.grid  # marker positions in lattice units
 
 
 # ------------------------------------------------------------------ concentration
