@@ -233,9 +233,12 @@ def substep_case(glsm, oracle, storage, per, pos):
 
 # ------------------------------------------------------------------ concentration
 def test_concentration_step_bit_exact(glsm, oracle):
+    """The scalar step (sampling, Robin source spreading, upwind advection,
+    diffusion) with walls and periodic faces, for a central gel and for gels
+    whose stencils are clipped by the walls or wrap across the faces."""
     say("concentration step")
-    for per in (0, 1):
-        g, og = glsm.Gel((6, 6, 6), (10, 10, 10)), oracle.Gel((6, 6, 6), (10, 10, 10))
+    for per, pos in ((0, (10, 10, 10)), (1, (10, 10, 10)), (0, (3.4, 10.0, 16.6)), (1, (1.0, 10.0, 19.5))):
+        g, og = glsm.Gel((6, 6, 6), pos), oracle.Gel((6, 6, 6), pos)
         c, oc = glsm.Coupler([g]), oracle.Coupler([og], 32)
         f, of = glsm.Fluid((20, 20, 20), c), oracle.Fluid((20, 20, 20), oc)
         for fl in (f, of):
