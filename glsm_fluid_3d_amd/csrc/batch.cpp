@@ -194,7 +194,6 @@ void Batch::rebuild() {
     b.g = glsm::fluid_geom(f);
     b.p = glsm::fluid_lbm_params(f);
     b.sp = glsm::ScalarParams{f.h_fp->dt, f.h_fp->D, f.h_fp->dx, f.h_fp->dy, f.h_fp->dz};
-    b.plane = f.plane;
     b.f[pf_] = f.d_f;
     b.f[pf_ ^ 1] = f.d_fnext;
     b.u = f.d_u;

@@ -171,7 +171,7 @@ void h2d(void* dev, const void* host, size_t bytes) {
 // device fields by their glsm_gel_field number plus the parameter block;
 // fluid records are the stored populations in their storage representation,
 // u, c and the parameter block.  A snapshot restores the state bit for bit.
-constexpr uint32_t kSnapVersion = 1;
+constexpr uint32_t kSnapVersion = 2;  // 2: distributions in the 256-cell block layout
 constexpr uint32_t kSnapGel = 1, kSnapFluid = 2;
 constexpr uint32_t kRecGelParams = 1000, kRecGelAnchor = 1001;
 constexpr uint32_t kRecFluidParams = 2000, kRecFluidRamp = 2001, kRecFluidF = 2002, kRecFluidU = 2003,
@@ -582,7 +582,7 @@ int glsm_fluid_conservation_static(glsm_fluid* f, double out[4]) {
   return guarded([&] {
     REQUIRE(out, "null output pointer");
     Fluid* p = F(f);
-    glsm::conservation_static(p->storage(), glsm::fluid_geom(*p), glsm::fluid_lbm_params(*p), p->d_f, p->plane, out,
+    glsm::conservation_static(p->storage(), glsm::fluid_geom(*p), glsm::fluid_lbm_params(*p), p->d_f, out,
                               p->fluid_stream);
     return GLSM_OK;
   });

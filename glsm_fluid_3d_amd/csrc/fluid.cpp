@@ -80,8 +80,9 @@ Fluid::Fluid(int3 size, int time, Coupler* cp)
   if (fluidNodeGrid.x < 2 || fluidNodeGrid.y < 2 || fluidNodeGrid.z < 2)
     throw std::invalid_argument("Fluid: lattice needs at least 2 nodes per axis");
   const long long Nll = (long long)fluidNodeGrid.x * fluidNodeGrid.y * fluidNodeGrid.z;
-  plane = (size_t)((Nll + 63) / 64 * 64);
-  if (19ull * plane >= (1ull << 32)) throw std::invalid_argument("Fluid: lattice too large for 32-bit plane indexing");
+  plane = glsm::dist_cells((size_t)Nll);
+  if (19ull * plane >= (1ull << 32))
+    throw std::invalid_argument("Fluid: lattice too large for 32-bit distribution indexing");
   N = (int)Nll;
   Nd = 19 * plane;
   threads = coupler->threads;
@@ -162,7 +163,7 @@ Fluid::Fluid(int3 size, int time, Coupler* cp)
   d_rtab_ = glsm::dalloc<float>(ent_cap_);
   d_wsum_ = glsm::dalloc<float>(std::max(1, coupler->sumGelBoundaryCount));
 
-  glsm::launch_lbm_init(storage_, geom(*this), lbm_params(*h_fp, storage_, scale_exp_), d_f, plane, d_u, d_c1, d_c2,
+  glsm::launch_lbm_init(storage_, geom(*this), lbm_params(*h_fp, storage_, scale_exp_), d_f, d_u, d_c1, d_c2,
                         fluid_stream);
   if (time) setGoonValue(time);
   GLSM_HIP(hipStreamSynchronize(fluid_stream));
@@ -311,7 +312,7 @@ void Fluid::setStorage(int storage, int scale_exp) {
   }
   float* tmp = glsm::dalloc<float>(Nd);
   glsm::launch_lbm_export(storage_, true, geom(*this), lbm_params(*h_fp, storage_, scale_exp_), d_f, tmp, plane,
-                          plane, fluid_stream);
+                          fluid_stream);
   GLSM_HIP(hipStreamSynchronize(fluid_stream));
   glsm::dfree(d_f);
   glsm::dfree(d_fnext);
@@ -320,7 +321,7 @@ void Fluid::setStorage(int storage, int scale_exp) {
   scale_exp_ = scale_exp;
   d_f = glsm::dalloc<char>(Nd * glsm::storage_bytes(storage_));
   d_fnext = glsm::dalloc<char>(Nd * glsm::storage_bytes(storage_));
-  glsm::launch_lbm_import(storage_, geom(*this), lbm_params(*h_fp, storage_, scale_exp_), tmp, d_f, plane, plane,
+  glsm::launch_lbm_import(storage_, geom(*this), lbm_params(*h_fp, storage_, scale_exp_), tmp, d_f, plane,
                           fluid_stream);
   GLSM_HIP(hipStreamSynchronize(fluid_stream));
   glsm::dfree(tmp);
@@ -368,7 +369,7 @@ void Fluid::writeFiles(int iter) {
 
 void Fluid::conservation(double out[4]) {
   double* dout = d_red_ + 4ull * nblk_;
-  glsm::launch_lbm_moments(storage_, geom(*this), lbm_params(*h_fp, storage_, scale_exp_), d_f, plane, d_red_, dout,
+  glsm::launch_lbm_moments(storage_, geom(*this), lbm_params(*h_fp, storage_, scale_exp_), d_f, d_red_, dout,
                            fluid_stream);
   GLSM_HIP(hipMemcpyAsync(out, dout, 4 * sizeof(double), hipMemcpyDeviceToHost, fluid_stream));
   GLSM_HIP(hipStreamSynchronize(fluid_stream));
@@ -377,7 +378,7 @@ void Fluid::conservation(double out[4]) {
 long long Fluid::checkFinite(double* max_abs_dev, double* limit) {
   // d_red_ holds 4 * nblk_ + 4 >= 8 doubles: the first two serve as counters
   auto* out2 = reinterpret_cast<unsigned long long*>(d_red_);
-  glsm::launch_lbm_health(storage_, geom(*this), lbm_params(*h_fp, storage_, scale_exp_), d_f, plane, d_u, d_c1, out2,
+  glsm::launch_lbm_health(storage_, geom(*this), lbm_params(*h_fp, storage_, scale_exp_), d_f, d_u, d_c1, out2,
                           fluid_stream);
   unsigned long long h[2] = {0, 0};
   GLSM_HIP(hipMemcpyAsync(h, out2, sizeof h, hipMemcpyDeviceToHost, fluid_stream));
@@ -399,7 +400,7 @@ long long Fluid::checkFinite(double* max_abs_dev, double* limit) {
 
 void Fluid::readDensity(double* host) {
   double* tmp = glsm::dalloc<double>(N);
-  glsm::launch_lbm_density(storage_, geom(*this), lbm_params(*h_fp, storage_, scale_exp_), d_f, plane, tmp,
+  glsm::launch_lbm_density(storage_, geom(*this), lbm_params(*h_fp, storage_, scale_exp_), d_f, tmp,
                            fluid_stream);
   GLSM_HIP(hipMemcpyAsync(host, tmp, N * sizeof(double), hipMemcpyDeviceToHost, fluid_stream));
   GLSM_HIP(hipStreamSynchronize(fluid_stream));
@@ -409,7 +410,7 @@ void Fluid::readDensity(double* host) {
 void Fluid::readPopulations(float* host) {
   float* tmp = glsm::dalloc<float>(Nd);
   glsm::launch_lbm_export(storage_, true, geom(*this), lbm_params(*h_fp, storage_, scale_exp_), d_f, tmp, plane,
-                          plane, fluid_stream);
+                          fluid_stream);
   GLSM_HIP(hipMemcpy2DAsync(host, N * sizeof(float), tmp, plane * sizeof(float), N * sizeof(float), 19,
                             hipMemcpyDeviceToHost, fluid_stream));
   GLSM_HIP(hipStreamSynchronize(fluid_stream));
@@ -420,7 +421,7 @@ void Fluid::writePopulations(const float* host) {
   float* tmp = glsm::dalloc<float>(Nd);
   GLSM_HIP(hipMemcpy2DAsync(tmp, plane * sizeof(float), host, N * sizeof(float), N * sizeof(float), 19,
                             hipMemcpyHostToDevice, fluid_stream));
-  glsm::launch_lbm_import(storage_, geom(*this), lbm_params(*h_fp, storage_, scale_exp_), tmp, d_f, plane, plane,
+  glsm::launch_lbm_import(storage_, geom(*this), lbm_params(*h_fp, storage_, scale_exp_), tmp, d_f, plane,
                           fluid_stream);
   GLSM_HIP(hipStreamSynchronize(fluid_stream));
   glsm::dfree(tmp);
@@ -429,7 +430,7 @@ void Fluid::writePopulations(const float* host) {
 void Fluid::readPostCollision(float* host) {
   float* tmp = glsm::dalloc<float>(Nd);
   glsm::launch_lbm_export(storage_, false, geom(*this), lbm_params(*h_fp, storage_, scale_exp_), d_f, tmp, plane,
-                          plane, fluid_stream);
+                          fluid_stream);
   GLSM_HIP(hipMemcpy2DAsync(host, N * sizeof(float), tmp, plane * sizeof(float), N * sizeof(float), 19,
                             hipMemcpyDeviceToHost, fluid_stream));
   GLSM_HIP(hipStreamSynchronize(fluid_stream));

@@ -202,6 +202,24 @@ void launch_ibm_sample(const MarkerArrays& m, const SpreadLists& sl, const float
 
 // distribution storage representations (collision is FP32 in all of them)
 enum StorageMode { F32_FULL = 0, F32_DEV = 1, F16_DEV = 2, BF16_DEV = 3 };
+
+// Layout of a distribution buffer: blocks of kDistBlock consecutive cells,
+// each holding its cells' 19 populations as 19 consecutive runs of
+// kDistBlock values.  Element (cell i, population q) is at
+//   (i / 256) * 19 * 256 + q * 256 + i % 256.
+// A 256-cell work-group of the stream/collide kernel then stores one
+// contiguous 19 * 256-value region and reads its pulled values from a few
+// such regions, instead of 19 runs in 19 planes a lattice apart (measured on
+// the pull pattern alone: 6% faster in FP32, 7.5% in FP16 at 201^3).  A
+// buffer holds `plane` = the cell count rounded up to whole blocks, times 19
+// values; every element index fits in 32 bits (checked at allocation).
+constexpr unsigned kDistBlock = 256;
+__host__ __device__ __forceinline__ unsigned dist_index(unsigned i, unsigned q) {
+  return i + (i / kDistBlock) * (18u * kDistBlock) + q * kDistBlock;
+}
+__host__ __device__ __forceinline__ size_t dist_cells(size_t n) {  // n rounded up to whole blocks
+  return (n + kDistBlock - 1) / kDistBlock * kDistBlock;
+}
 size_t storage_bytes(int storage);
 // device copy of `bytes` (multiple of 16), 16 B per lane
 void launch_copy16(const void* a, void* b, size_t bytes, hipStream_t s);
@@ -222,22 +240,23 @@ void launch_lbm_step(int storage, const LatticeGeom& g, const LbmParams& p, cons
 // (-1 = automatic: 4 when the state streams from HBM, else 0); a
 // measurement switch, process-wide
 void set_lattice_xcd_run(int run);
-void launch_lbm_init(int storage, const LatticeGeom& g, const LbmParams& p, void* f, size_t plane,
-                     float3* u, float* c1, float* c2, hipStream_t s);
+void launch_lbm_init(int storage, const LatticeGeom& g, const LbmParams& p, void* f, float3* u, float* c1,
+                     float* c2, hipStream_t s);
 // stored state -> FP32 full populations (pull = pre-collision of the next
-// sub-step, else the stored post-collision ones), and the inverse of pull
+// sub-step, else the stored post-collision ones), and the inverse of pull.
+// The FP32 side is plane-major: population q of cell i at q * out_plane + i.
 void launch_lbm_export(int storage, bool pull, const LatticeGeom& g, const LbmParams& p, const void* fpost,
-                       float* out, size_t plane, size_t out_plane, hipStream_t s);
+                       float* out, size_t out_plane, hipStream_t s);
 // FP64 sum(rho - 1) and sum(rho u) of the pre-collision populations into
 // out4[0..3]; `partial` holds 4 doubles per 256-cell block
-void launch_lbm_moments(int storage, const LatticeGeom& g, const LbmParams& p, const void* fpost, size_t plane,
+void launch_lbm_moments(int storage, const LatticeGeom& g, const LbmParams& p, const void* fpost,
                         double* partial, double* out4, hipStream_t s);
 void launch_lbm_import(int storage, const LatticeGeom& g, const LbmParams& p, const float* in, void* fpost,
-                       size_t plane, size_t in_plane, hipStream_t s);
+                       size_t in_plane, hipStream_t s);
 // health scan of a lattice: out2[0] = cells with a non-finite stored value,
 // velocity or concentration; out2[1] = bits of the largest finite
 // |f_q - w_q| (a non-negative double).  out2 is cleared first.
-void launch_lbm_health(int storage, const LatticeGeom& g, const LbmParams& p, const void* fpost, size_t plane,
+void launch_lbm_health(int storage, const LatticeGeom& g, const LbmParams& p, const void* fpost,
                        const float3* u, const float* c, unsigned long long* out2, hipStream_t s);
 // p[0..n) = 0, and *out += number of non-finite values of a[0..n)
 void launch_zero_u64(unsigned long long* p, int n, hipStream_t s);
@@ -249,10 +268,10 @@ void launch_count_nonfinite(const double* a, size_t n, unsigned long long* out, 
 // current device (once per device).
 double* static_scratch();
 long static_scratch_max_cells();
-void conservation_static(int storage, const LatticeGeom& g, const LbmParams& p, const void* fpost, size_t plane,
+void conservation_static(int storage, const LatticeGeom& g, const LbmParams& p, const void* fpost,
                          double* host_out4, hipStream_t s);
 // rho - 1 per cell (FP64) of the pre-collision populations of the next sub-step
-void launch_lbm_density(int storage, const LatticeGeom& g, const LbmParams& p, const void* fpost, size_t plane,
+void launch_lbm_density(int storage, const LatticeGeom& g, const LbmParams& p, const void* fpost,
                         double* out, hipStream_t s);
 
 struct ScalarParams {
@@ -283,8 +302,7 @@ struct BatchInst {
   LatticeGeom g;
   LbmParams p;
   ScalarParams sp;
-  size_t plane;
-  void* f[2];     // the two post-collision buffers (storage representation)
+  void* f[2];     // the two post-collision buffers (storage representation, block layout)
   float3* u;
   float* c[2];    // concentration ping-pong
   SpreadLists sl;
@@ -337,16 +355,13 @@ void launch_advance_iters(int* iters, int n, hipStream_t s);
 // ---- z-slab decomposition (slab_kernels.hip): copies of the stored
 // representation, `elem` = 2 or 4 bytes per value ----
 // slab planes 0..nz+1 <- lattice planes z0-1..z0+nz (wrapped if perZ)
-void launch_slab_gather(size_t elem, const void* src, size_t splane, void* dst, size_t dplane, size_t nxy, int z0,
-                        int nz, int Nz, int perZ, hipStream_t s);
+void launch_slab_gather(size_t elem, const void* src, void* dst, size_t nxy, int z0, int nz, int Nz, int perZ,
+                        hipStream_t s);
 // lattice planes z0..z0+nz-1 <- slab planes 1..nz
-void launch_slab_scatter(size_t elem, const void* src, size_t splane, void* dst, size_t dplane, size_t nxy, int z0,
-                         int nz, hipStream_t s);
+void launch_slab_scatter(size_t elem, const void* src, void* dst, size_t nxy, int z0, int nz, hipStream_t s);
 // face buffer (5 x nxy) <-> the c_z = +1 (which = 0) or c_z = -1 (which = 1)
 // populations of slab plane z
-void launch_face_pack(size_t elem, const void* f, size_t plane, size_t nxy, int z, int which, void* out,
-                      hipStream_t s);
-void launch_face_unpack(size_t elem, const void* in, void* f, size_t plane, size_t nxy, int z, int which,
-                        hipStream_t s);
+void launch_face_pack(size_t elem, const void* f, size_t nxy, int z, int which, void* out, hipStream_t s);
+void launch_face_unpack(size_t elem, const void* in, void* f, size_t nxy, int z, int which, hipStream_t s);
 
 }  // namespace glsm

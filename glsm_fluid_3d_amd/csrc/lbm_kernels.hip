@@ -5,10 +5,11 @@
 // (k_set_force, k_vec_add, k_macros, k_collide, k_stream_bounce,
 // k_zero_vector + the swap, fluid.cu:273-282, ~496 B/cell).  Each thread owns
 // one cell and
-//   1. pulls f_k(x,q) = fpost(x - c_q, q) from the 19 structure-of-arrays
-//      planes, or fpost(x, opp q) when x - c_q lies outside a wall -- the
-//      pull form of the reference's push with halfway bounce-back
-//      (fluid_kernels.cu:144-165); a periodic axis wraps instead;
+//   1. pulls f_k(x,q) = fpost(x - c_q, q), or fpost(x, opp q) when x - c_q
+//      lies outside a wall -- the pull form of the reference's push with
+//      halfway bounce-back (fluid_kernels.cu:144-165); a periodic axis wraps
+//      instead.  The buffers are in the 256-cell block layout (dist_index):
+//      a work-group's cells store one contiguous region;
 //   2. forms rho, u = (sum f c + F/2)/rho with F = F_const + F_ibm, where
 //      F_ibm is gathered from the marker forces through the cell's sorted
 //      (marker, weight) list -- only in 256-cell blocks that some marker
@@ -19,8 +20,8 @@
 // type (152 B in FP32, 76 B in FP16/BF16); u (12 B) is stored only in the
 // cells some marker stencil covers (those the interpolation reads) and, for
 // every cell, on the last sub-step.  Adjacent lanes own adjacent cells, so
-// every plane access of a 64-lane wavefront is one contiguous run (shifted
-// by the stream offset).
+// every population access of a 64-lane wavefront is one contiguous run
+// (shifted by the stream offset, and split where it crosses a block).
 //
 // Storage policies (collision always in FP32):
 //   F32_FULL : f itself, the reference's representation;
@@ -175,9 +176,10 @@ struct PullAddr {
     kzp = zl || g.perZ; ozp = zl ? -sz : (g.Nz - 1) * sz;
     kzm = zh || g.perZ; ozm = zh ? sz : -(g.Nz - 1) * sz;
   }
-  // flat index of the source of population q arriving at cell i
+  // distribution element (dist_index) of the source of population q
+  // arriving at cell i
   template <int q>
-  __device__ __forceinline__ unsigned src(unsigned i, unsigned plane) const {
+  __device__ __forceinline__ unsigned src(unsigned i) const {
     bool ok = true;
     int off = 0;
     if (CX[q] > 0) { ok = ok && kxp; off += oxp; }
@@ -186,10 +188,15 @@ struct PullAddr {
     if (CY[q] < 0) { ok = ok && kym; off += oym; }
     if (CZ[q] > 0) { ok = ok && kzp; off += ozp; }
     if (CZ[q] < 0) { ok = ok && kzm; off += ozm; }
-    return ok ? (unsigned)q * plane + (unsigned)((int)i + off) : (unsigned)OPP[q] * plane + i;
+    // (cell, population) = (i + off, q), or (i, opp q) at a wall: the cell
+    // part is selected and q's run offset added after, a uniform term
+    const unsigned j = (unsigned)((int)i + off);
+    const unsigned cell = ok ? dist_index(j, 0) : dist_index(i, 0) + (unsigned)(OPP[q] - q) * kDistBlock;
+    return cell + (unsigned)q * kDistBlock;
   }
-  // flat index of the destination of population q leaving cell i (push):
-  // the cell it streams into, or (i, opp q) when it bounces off a wall
+  // index into a plane-major array (population q of cell i at q * plane + i)
+  // of the destination of population q leaving cell i (push): the cell it
+  // streams into, or (i, opp q) when it bounces off a wall
   template <int q>
   __device__ __forceinline__ unsigned dst(unsigned i, unsigned plane) const {
     bool ok = true;
@@ -245,8 +252,8 @@ __device__ __forceinline__ T* elem_at(T* f, unsigned idx) {
 // accumulating 19 roundings per sub-step.
 template <int S, bool FORCED, bool NT, bool B32>
 __device__ __forceinline__ void collide_store(const float (&v)[19], float Fx, float Fy, float Fz, const LbmParams& p,
-                                              GLSM_GLOBAL typename Store<S>::T* __restrict__ fout, unsigned plane,
-                                              unsigned i, GLSM_GLOBAL Vec3* __restrict__ u, bool store_u) {
+                                              GLSM_GLOBAL typename Store<S>::T* __restrict__ fout, unsigned i,
+                                              GLSM_GLOBAL Vec3* __restrict__ u, bool store_u) {
   constexpr bool DEV = Store<S>::dev;
   constexpr bool MOM64 = S == F16_DEV || S == BF16_DEV;
   constexpr bool SCALED = Store<S>::scaled;
@@ -314,10 +321,10 @@ __device__ __forceinline__ void collide_store(const float (&v)[19], float Fx, fl
       const float ciF = cx * Fx + cy * Fy + cz * Fz;
       out = fmaf(SCALED ? w * kF * sc : w * kF, fmaf(9.0f * cu, ciF, 3.0f * (ciF - uF)), out);
     }
-    const float st = Store<S>::template st<NT>(elem_at<B32>(fout, (unsigned)q * plane + i), out);
+    const float st = Store<S>::template st<NT>(elem_at<B32>(fout, dist_index(i, q)), out);
     if (MOM64) kept += st;
   });
-  if (MOM64) Store<S>::template st<NT>(elem_at<B32>(fout, i), (float)(sd - (double)kept));
+  if (MOM64) Store<S>::template st<NT>(elem_at<B32>(fout, dist_index(i, 0)), (float)(sd - (double)kept));
 }
 
 // Every access goes through GLSM_GLOBAL pointers: the batched kernels take
@@ -326,8 +333,8 @@ __device__ __forceinline__ void collide_store(const float (&v)[19], float Fx, fl
 template <int S, bool NT, bool B32>
 __device__ __forceinline__ void lbm_block(const LatticeGeom& g, const LbmParams& p,
                                           const typename Store<S>::T* __restrict__ fin,
-                                          typename Store<S>::T* __restrict__ fout, unsigned plane,
-                                          float3* __restrict__ u, int write_u_all, const SpreadLists& sl,
+                                          typename Store<S>::T* __restrict__ fout, float3* __restrict__ u,
+                                          int write_u_all, const SpreadLists& sl,
                                           const float3* __restrict__ Fl, int const_force, unsigned b,
                                           unsigned lane) {
   const unsigned i = b * kThreads + lane;
@@ -340,7 +347,7 @@ __device__ __forceinline__ void lbm_block(const LatticeGeom& g, const LbmParams&
   float v[19];
   Unroll<19>::run([&](auto qc) {
     constexpr int q = decltype(qc)::value;
-    v[q] = Store<S>::ld(elem_at<B32>(gin, a.src<q>(i, plane)));
+    v[q] = Store<S>::ld(elem_at<B32>(gin, a.src<q>(i)));
   });
   bool store_u = write_u_all;
   GLSM_GLOBAL Vec3* gu = gmem(reinterpret_cast<Vec3*>(u));
@@ -382,9 +389,9 @@ __device__ __forceinline__ void lbm_block(const LatticeGeom& g, const LbmParams&
           }
       }
     }
-    collide_store<S, true, NT, B32>(v, p.Fc.x + Fx, p.Fc.y + Fy, p.Fc.z + Fz, p, gmem(fout), plane, i, gu, store_u);
+    collide_store<S, true, NT, B32>(v, p.Fc.x + Fx, p.Fc.y + Fy, p.Fc.z + Fz, p, gmem(fout), i, gu, store_u);
   } else {
-    collide_store<S, false, NT, B32>(v, 0.f, 0.f, 0.f, p, gmem(fout), plane, i, gu, store_u);
+    collide_store<S, false, NT, B32>(v, 0.f, 0.f, 0.f, p, gmem(fout), i, gu, store_u);
   }
 }
 
@@ -393,11 +400,11 @@ __device__ __forceinline__ void lbm_block(const LatticeGeom& g, const LbmParams&
 template <int S, bool NT, bool B32, int RUN>
 __global__ void __launch_bounds__(kThreads, 8) k_lbm_step(LatticeGeom g, LbmParams p,
                                                        const typename Store<S>::T* __restrict__ fin,
-                                                       typename Store<S>::T* __restrict__ fout, unsigned plane,
+                                                       typename Store<S>::T* __restrict__ fout,
                                                        float3* __restrict__ u, int write_u_all, SpreadLists sl,
                                                        const float3* __restrict__ Fl, int const_force) {
   const unsigned b = xcd_block<RUN>(blockIdx.x, gridDim.x);
-  lbm_block<S, NT, B32>(g, p, fin, fout, plane, u, write_u_all, sl, Fl, const_force, b, threadIdx.x);
+  lbm_block<S, NT, B32>(g, p, fin, fout, u, write_u_all, sl, Fl, const_force, b, threadIdx.x);
 }
 
 // All members of a batch: the blocks of every lattice form one flattened
@@ -411,8 +418,7 @@ __global__ void __launch_bounds__(kThreads, 8) kb_lbm_step(BatchTables t, int se
     const unsigned gb = xcd_block<RUN>(w, (unsigned)t.nblk);
     const BatchInst& I = t.inst[t.blk_inst[gb]];
     lbm_block<S, NT, true>(I.g, I.p, static_cast<const T*>(I.f[sel]), static_cast<T*>(I.f[sel ^ 1]),
-                           (unsigned)I.plane, I.u, write_u_all, I.sl, I.m.Fdrag, I.const_force,
-                           gb - (unsigned)I.blk0, threadIdx.x);
+                           I.u, write_u_all, I.sl, I.m.Fdrag, I.const_force, gb - (unsigned)I.blk0, threadIdx.x);
   }
 }
 
@@ -434,8 +440,8 @@ __global__ void __launch_bounds__(kThreads * WGB, 8) kb_lbm_step_units(BatchTabl
     if (gb >= (unsigned)t.nblk) break;
     const BatchInst& I = t.inst[t.blk_inst[gb]];
     lbm_block<S, NT, true>(I.g, I.p, static_cast<const T*>(I.f[sel]), static_cast<T*>(I.f[sel ^ 1]),
-                           (unsigned)I.plane, I.u, write_u_all, I.sl, I.m.Fdrag, I.const_force,
-                           gb - (unsigned)I.blk0, threadIdx.x % kThreads);
+                           I.u, write_u_all, I.sl, I.m.Fdrag, I.const_force, gb - (unsigned)I.blk0,
+                           threadIdx.x % kThreads);
   }
 }
 
@@ -448,14 +454,14 @@ __global__ void k_advance_iters(int* __restrict__ iters, int n) {
 // post-collision state), u = 0, c1 = c2 = 0
 template <int S>
 __global__ void __launch_bounds__(kThreads) k_lbm_init(LatticeGeom g, LbmParams p,
-                                                       typename Store<S>::T* __restrict__ f, unsigned plane,
+                                                       typename Store<S>::T* __restrict__ f,
                                                        float3* __restrict__ u, float* __restrict__ c1,
                                                        float* __restrict__ c2) {
   const unsigned i = blockIdx.x * kThreads + threadIdx.x;
   if (i >= (unsigned)g.N) return;
 #pragma unroll
   for (int q = 0; q < 19; ++q)
-    Store<S>::template st<false>(f + (unsigned)q * plane + i, Store<S>::dev ? 0.f : wq(p, q));
+    Store<S>::template st<false>(f + dist_index(i, q), Store<S>::dev ? 0.f : wq(p, q));
   u[i] = make_float3(0.f, 0.f, 0.f);
   c1[i] = 0.f;
   c2[i] = 0.f;
@@ -466,7 +472,7 @@ __global__ void __launch_bounds__(kThreads) k_lbm_init(LatticeGeom g, LbmParams 
 template <int S, bool PULL>
 __global__ void __launch_bounds__(kThreads) k_lbm_export(LatticeGeom g, LbmParams p,
                                                          const typename Store<S>::T* __restrict__ fpost,
-                                                         float* __restrict__ out, unsigned plane, unsigned out_plane) {
+                                                         float* __restrict__ out, unsigned out_plane) {
   const unsigned i = blockIdx.x * kThreads + threadIdx.x;
   if (i >= (unsigned)g.N) return;
   int x, y, z;
@@ -474,7 +480,7 @@ __global__ void __launch_bounds__(kThreads) k_lbm_export(LatticeGeom g, LbmParam
   const PullAddr a(g, x, y, z);
   Unroll<19>::run([&](auto qc) {
     constexpr int q = decltype(qc)::value;
-    const unsigned s = PULL ? a.src<q>(i, plane) : (unsigned)q * plane + i;
+    const unsigned s = PULL ? a.src<q>(i) : dist_index(i, q);
     const float v = to_value<S>(Store<S>::ld(fpost + s), p);
     out[(unsigned)q * out_plane + i] = Store<S>::dev ? wq(p, q) + v : v;
   });
@@ -484,8 +490,7 @@ __global__ void __launch_bounds__(kThreads) k_lbm_export(LatticeGeom g, LbmParam
 // a wall, converted to the storage representation
 template <int S>
 __global__ void __launch_bounds__(kThreads) k_lbm_import(LatticeGeom g, LbmParams p, const float* __restrict__ in,
-                                                         typename Store<S>::T* __restrict__ fpost, unsigned plane,
-                                                         unsigned in_plane) {
+                                                         typename Store<S>::T* __restrict__ fpost, unsigned in_plane) {
   const unsigned i = blockIdx.x * kThreads + threadIdx.x;
   if (i >= (unsigned)g.N) return;
   int x, y, z;
@@ -494,7 +499,7 @@ __global__ void __launch_bounds__(kThreads) k_lbm_import(LatticeGeom g, LbmParam
   Unroll<19>::run([&](auto qc) {
     constexpr int q = decltype(qc)::value;
     const float v = in[a.dst<q>(i, in_plane)];  // from plane q or opp q; w_opp(q) == w_q
-    Store<S>::template st<false>(fpost + (unsigned)q * plane + i, to_raw<S>(Store<S>::dev ? v - wq(p, q) : v, p));
+    Store<S>::template st<false>(fpost + dist_index(i, q), to_raw<S>(Store<S>::dev ? v - wq(p, q) : v, p));
   });
 }
 
@@ -532,7 +537,7 @@ __device__ __forceinline__ void block_sum_store(double (&acc)[K], double* __rest
 template <int S>
 __global__ void __launch_bounds__(kThreads) k_lbm_moments(LatticeGeom g, LbmParams p,
                                                           const typename Store<S>::T* __restrict__ fpost,
-                                                          unsigned plane, double* __restrict__ partial) {
+                                                          double* __restrict__ partial) {
   const unsigned i = blockIdx.x * kThreads + threadIdx.x;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   if (i < (unsigned)g.N) {
@@ -542,7 +547,7 @@ __global__ void __launch_bounds__(kThreads) k_lbm_moments(LatticeGeom g, LbmPara
     double m = Store<S>::dev ? 0.0 : -1.0;
     Unroll<19>::run([&](auto qc) {
       constexpr int q = decltype(qc)::value;
-      const double f = (double)to_value<S>(Store<S>::ld(fpost + a.src<q>(i, plane)), p);
+      const double f = (double)to_value<S>(Store<S>::ld(fpost + a.src<q>(i)), p);
       // deviation storage: sum d = rho - 1 (the weights sum to 1), and
       // sum d c = sum f c (the weights of opposite velocities are equal)
       m += f;
@@ -587,7 +592,7 @@ __device__ __forceinline__ double wave_max(double v) {
 template <int S>
 __global__ void __launch_bounds__(kThreads) k_lbm_health(LatticeGeom g, LbmParams p,
                                                          const typename Store<S>::T* __restrict__ fpost,
-                                                         unsigned plane, const float3* __restrict__ u,
+                                                         const float3* __restrict__ u,
                                                          const float* __restrict__ c,
                                                          unsigned long long* __restrict__ out2) {
   const unsigned i = blockIdx.x * kThreads + threadIdx.x;
@@ -597,7 +602,7 @@ __global__ void __launch_bounds__(kThreads) k_lbm_health(LatticeGeom g, LbmParam
     bool ok = true;
 #pragma unroll
     for (int q = 0; q < 19; ++q) {
-      const float v = to_value<S>(Store<S>::ld(fpost + (unsigned)q * plane + i), p);
+      const float v = to_value<S>(Store<S>::ld(fpost + dist_index(i, q)), p);
       const float d = Store<S>::dev ? v : v - wq(p, q);
       if (__builtin_isfinite(d))
         mx = fmax(mx, fabs((double)d));
@@ -637,7 +642,7 @@ __global__ void k_fill_u64(unsigned long long* __restrict__ p, int n) {
 template <int S>
 __global__ void __launch_bounds__(kThreads) k_lbm_density(LatticeGeom g, LbmParams p,
                                                           const typename Store<S>::T* __restrict__ fpost,
-                                                          unsigned plane, double* __restrict__ out) {
+                                                          double* __restrict__ out) {
   const unsigned i = blockIdx.x * kThreads + threadIdx.x;
   if (i >= (unsigned)g.N) return;
   int x, y, z;
@@ -646,7 +651,7 @@ __global__ void __launch_bounds__(kThreads) k_lbm_density(LatticeGeom g, LbmPara
   double m = Store<S>::dev ? 0.0 : -1.0;
   Unroll<19>::run([&](auto qc) {
     constexpr int q = decltype(qc)::value;
-    m += (double)to_value<S>(Store<S>::ld(fpost + a.src<q>(i, plane)), p);
+    m += (double)to_value<S>(Store<S>::ld(fpost + a.src<q>(i)), p);
   });
   out[i] = m;
 }
@@ -690,8 +695,7 @@ void step_impl(const LatticeGeom& g, const LbmParams& p, const void* fin, void* 
   const int wu = write_u_all ? 1 : 0;
   const bool b32 = fits_b32(plane, sizeof(T));
   auto go = [&](auto kernel) {
-    launch_maybe_timed(kernel, blocks_for(g.N), s, ev0, ev1, g, p, in, out, (unsigned)plane, u, wu, sl, Fl,
-                       const_force);
+    launch_maybe_timed(kernel, blocks_for(g.N), s, ev0, ev1, g, p, in, out, u, wu, sl, Fl, const_force);
   };
   // XCD run length: 4-block runs once the state streams from HBM, one
   // contiguous run per XCD while it stays in the Infinity Cache, unless
@@ -822,45 +826,43 @@ void launch_advance_iters(int* iters, int n, hipStream_t s) {
   GLSM_LAUNCH_CHECK();
 }
 
-void launch_lbm_init(int storage, const LatticeGeom& g, const LbmParams& p, void* f, size_t plane, float3* u,
-                     float* c1, float* c2, hipStream_t s) {
+void launch_lbm_init(int storage, const LatticeGeom& g, const LbmParams& p, void* f, float3* u, float* c1,
+                     float* c2, hipStream_t s) {
   GLSM_STORAGE_DISPATCH(storage, (k_lbm_init<S><<<blocks_for(g.N), kThreads, 0, s>>>(
-                                     g, p, static_cast<typename Store<S>::T*>(f), (unsigned)plane, u, c1, c2)));
+                                     g, p, static_cast<typename Store<S>::T*>(f), u, c1, c2)));
   GLSM_LAUNCH_CHECK();
 }
 
 void launch_lbm_export(int storage, bool pull, const LatticeGeom& g, const LbmParams& p, const void* fpost,
-                       float* out, size_t plane, size_t out_plane, hipStream_t s) {
+                       float* out, size_t out_plane, hipStream_t s) {
   if (pull) {
     GLSM_STORAGE_DISPATCH(storage, (k_lbm_export<S, true><<<blocks_for(g.N), kThreads, 0, s>>>(
-                                       g, p, static_cast<const typename Store<S>::T*>(fpost), out, (unsigned)plane,
+                                       g, p, static_cast<const typename Store<S>::T*>(fpost), out,
                                        (unsigned)out_plane)));
   } else {
     GLSM_STORAGE_DISPATCH(storage, (k_lbm_export<S, false><<<blocks_for(g.N), kThreads, 0, s>>>(
-                                       g, p, static_cast<const typename Store<S>::T*>(fpost), out, (unsigned)plane,
+                                       g, p, static_cast<const typename Store<S>::T*>(fpost), out,
                                        (unsigned)out_plane)));
   }
   GLSM_LAUNCH_CHECK();
 }
 
-void launch_lbm_moments(int storage, const LatticeGeom& g, const LbmParams& p, const void* fpost, size_t plane,
+void launch_lbm_moments(int storage, const LatticeGeom& g, const LbmParams& p, const void* fpost,
                         double* partial, double* out4, hipStream_t s) {
   const int nb = blocks_for(g.N);
   GLSM_STORAGE_DISPATCH(storage, (k_lbm_moments<S><<<nb, kThreads, 0, s>>>(
-                                     g, p, static_cast<const typename Store<S>::T*>(fpost), (unsigned)plane,
-                                     partial)));
+                                     g, p, static_cast<const typename Store<S>::T*>(fpost), partial)));
   GLSM_LAUNCH_CHECK();
   k_reduce_partials<<<1, kThreads, 0, s>>>(partial, nb, out4);
   GLSM_LAUNCH_CHECK();
 }
 
-void launch_lbm_health(int storage, const LatticeGeom& g, const LbmParams& p, const void* fpost, size_t plane,
+void launch_lbm_health(int storage, const LatticeGeom& g, const LbmParams& p, const void* fpost,
                        const float3* u, const float* c, unsigned long long* out2, hipStream_t s) {
   k_fill_u64<<<1, 64, 0, s>>>(out2, 2);
   GLSM_LAUNCH_CHECK();
   GLSM_STORAGE_DISPATCH(storage, (k_lbm_health<S><<<blocks_for(g.N), kThreads, 0, s>>>(
-                                     g, p, static_cast<const typename Store<S>::T*>(fpost), (unsigned)plane, u, c,
-                                     out2)));
+                                     g, p, static_cast<const typename Store<S>::T*>(fpost), u, c, out2)));
   GLSM_LAUNCH_CHECK();
 }
 
@@ -876,10 +878,10 @@ void launch_count_nonfinite(const double* a, size_t n, unsigned long long* out, 
   GLSM_LAUNCH_CHECK();
 }
 
-void launch_lbm_density(int storage, const LatticeGeom& g, const LbmParams& p, const void* fpost, size_t plane,
+void launch_lbm_density(int storage, const LatticeGeom& g, const LbmParams& p, const void* fpost,
                         double* out, hipStream_t s) {
   GLSM_STORAGE_DISPATCH(storage, (k_lbm_density<S><<<blocks_for(g.N), kThreads, 0, s>>>(
-                                     g, p, static_cast<const typename Store<S>::T*>(fpost), (unsigned)plane, out)));
+                                     g, p, static_cast<const typename Store<S>::T*>(fpost), out)));
   GLSM_LAUNCH_CHECK();
 }
 
@@ -915,7 +917,7 @@ double* static_scratch() {
 
 long static_scratch_max_cells() { return (long)kStaticBlocks * kThreads; }
 
-void conservation_static(int storage, const LatticeGeom& g, const LbmParams& p, const void* fpost, size_t plane,
+void conservation_static(int storage, const LatticeGeom& g, const LbmParams& p, const void* fpost,
                          double* host_out4, hipStream_t s) {
   const int nb = blocks_for(g.N);
   if (nb > kStaticBlocks)
@@ -925,8 +927,7 @@ void conservation_static(int storage, const LatticeGeom& g, const LbmParams& p, 
   double* partial = static_scratch();
   double* out4 = partial + 4ull * kStaticBlocks;
   GLSM_STORAGE_DISPATCH(storage, (k_lbm_moments<S><<<nb, kThreads, 0, s>>>(
-                                     g, p, static_cast<const typename Store<S>::T*>(fpost), (unsigned)plane,
-                                     partial)));
+                                     g, p, static_cast<const typename Store<S>::T*>(fpost), partial)));
   GLSM_LAUNCH_CHECK();
   k_reduce_partials<<<1, kThreads, 0, s>>>(partial, nb, out4);
   GLSM_LAUNCH_CHECK();
@@ -935,10 +936,9 @@ void conservation_static(int storage, const LatticeGeom& g, const LbmParams& p, 
 }
 
 void launch_lbm_import(int storage, const LatticeGeom& g, const LbmParams& p, const float* in, void* fpost,
-                       size_t plane, size_t in_plane, hipStream_t s) {
+                       size_t in_plane, hipStream_t s) {
   GLSM_STORAGE_DISPATCH(storage, (k_lbm_import<S><<<blocks_for(g.N), kThreads, 0, s>>>(
-                                     g, p, in, static_cast<typename Store<S>::T*>(fpost), (unsigned)plane,
-                                     (unsigned)in_plane)));
+                                     g, p, in, static_cast<typename Store<S>::T*>(fpost), (unsigned)in_plane)));
   GLSM_LAUNCH_CHECK();
 }
 

@@ -42,8 +42,8 @@ Slab::Slab(const Fluid& like, int z0, int nz) : dev_(new glsm::SlabDevice()) {
   g.zlo = wall_lo_ ? 1 : 0;
   g.zhi = wall_hi_ ? nz : nz + 1;
   dev_->p = glsm::fluid_lbm_params(like);
-  plane_ = ((size_t)g.N + 63) / 64 * 64;
-  if (19ull * plane_ >= (1ull << 32)) throw std::invalid_argument("Slab: too large for 32-bit plane indexing");
+  plane_ = glsm::dist_cells((size_t)g.N);
+  if (19ull * plane_ >= (1ull << 32)) throw std::invalid_argument("Slab: too large for 32-bit element indexing");
   const size_t elem = glsm::storage_bytes(storage_);
   for (auto& f : f_) f = glsm::dalloc<char>(19 * plane_ * elem);
   u_ = glsm::dalloc<float3>(g.N);
@@ -75,8 +75,8 @@ void Slab::load(const Fluid& src) {
   const glsm::LatticeGeom sg = glsm::fluid_geom(src);
   if (sg.Nx != Nx_ || sg.Ny != Ny_ || sg.Nz != Nz_ || src.storage() != storage_)
     throw std::invalid_argument("Slab::load: fluid has another lattice or storage");
-  glsm::launch_slab_gather(glsm::storage_bytes(storage_), src.d_f, src.plane, f_[cur_], plane_, (size_t)Nx_ * Ny_,
-                           z0_, nz_, Nz_, perZ_ ? 1 : 0, s_);
+  glsm::launch_slab_gather(glsm::storage_bytes(storage_), src.d_f, f_[cur_], (size_t)Nx_ * Ny_, z0_, nz_, Nz_,
+                           perZ_ ? 1 : 0, s_);
   GLSM_HIP(hipStreamSynchronize(s_));
 }
 
@@ -84,8 +84,7 @@ void Slab::store(Fluid& dst) const {
   const glsm::LatticeGeom dg = glsm::fluid_geom(dst);
   if (dg.Nx != Nx_ || dg.Ny != Ny_ || dg.Nz != Nz_ || dst.storage() != storage_)
     throw std::invalid_argument("Slab::store: fluid has another lattice or storage");
-  glsm::launch_slab_scatter(glsm::storage_bytes(storage_), f_[cur_], plane_, dst.d_f, dst.plane, (size_t)Nx_ * Ny_,
-                            z0_, nz_, s_);
+  glsm::launch_slab_scatter(glsm::storage_bytes(storage_), f_[cur_], dst.d_f, (size_t)Nx_ * Ny_, z0_, nz_, s_);
   GLSM_HIP(hipStreamSynchronize(s_));
 }
 
@@ -106,8 +105,8 @@ void Slab::step(bool write_u) {
 void Slab::pack() {
   const size_t elem = glsm::storage_bytes(storage_), nxy = (size_t)Nx_ * Ny_;
   if (timing_) GLSM_HIP(hipEventRecord(ev_[2], s_));
-  if (!wall_hi_) glsm::launch_face_pack(elem, f_[cur_], plane_, nxy, nz_, 0, send_up_, s_);
-  if (!wall_lo_) glsm::launch_face_pack(elem, f_[cur_], plane_, nxy, 1, 1, send_down_, s_);
+  if (!wall_hi_) glsm::launch_face_pack(elem, f_[cur_], nxy, nz_, 0, send_up_, s_);
+  if (!wall_lo_) glsm::launch_face_pack(elem, f_[cur_], nxy, 1, 1, send_down_, s_);
 }
 
 void Slab::receiveFromBelow(const Slab& below) {
@@ -126,8 +125,8 @@ void Slab::receiveFromAbove(const Slab& above) {
 
 void Slab::unpack() {
   const size_t elem = glsm::storage_bytes(storage_), nxy = (size_t)Nx_ * Ny_;
-  if (!wall_lo_) glsm::launch_face_unpack(elem, recv_below_, f_[cur_], plane_, nxy, 0, 0, s_);
-  if (!wall_hi_) glsm::launch_face_unpack(elem, recv_above_, f_[cur_], plane_, nxy, nz_ + 1, 1, s_);
+  if (!wall_lo_) glsm::launch_face_unpack(elem, recv_below_, f_[cur_], nxy, 0, 0, s_);
+  if (!wall_hi_) glsm::launch_face_unpack(elem, recv_above_, f_[cur_], nxy, nz_ + 1, 1, s_);
   if (timing_) {
     GLSM_HIP(hipEventRecord(ev_[3], s_));
     GLSM_HIP(hipEventSynchronize(ev_[3]));

@@ -132,8 +132,8 @@ class Fluid {
   float dt;
   float h;
   int N;
-  size_t Nd;        // 19 * plane stride
-  size_t plane;     // plane stride in floats (N rounded up to 64)
+  size_t Nd;        // values per distribution buffer: 19 * plane
+  size_t plane;     // cells rounded up to whole 256-cell layout blocks
   float dt_fluid;
   int Nsub;
   float niu;

@@ -72,7 +72,7 @@ class Slab {
  private:
   int Nx_, Ny_, Nz_, z0_, nz_, storage_, scale_exp_;
   bool perZ_, wall_lo_, wall_hi_;
-  size_t plane_;  // plane stride of the slab buffers (cells incl. halos, rounded to 64)
+  size_t plane_;  // cells of the slab buffers (halos included) rounded up to whole 256-cell layout blocks
   std::unique_ptr<glsm::SlabDevice> dev_;  // geometry and collision parameters
   void* f_[2] = {nullptr, nullptr};
   float3* u_ = nullptr;
