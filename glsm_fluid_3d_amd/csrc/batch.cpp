@@ -93,7 +93,7 @@ int Batch::add(std::vector<Gel*>& gels, Coupler* coupler, Fluid* fluid, int iter
   if (fluid->Nsub != nsub_) throw std::invalid_argument("Batch::add: members must share the sub-step count");
   // the batched stream/collide kernel addresses a member's distributions by
   // 32-bit byte offsets
-  if (19ull * fluid->plane * glsm::storage_bytes(storage_) >= (1ull << 32))
+  if (!glsm::dist_fits_b32(fluid->plane, glsm::storage_bytes(storage_)))
     throw std::invalid_argument("Batch::add: lattice too large for a batch member (distributions over 4 GiB)");
   if (fluid->storage() != storage_ || (storage_ == Fluid::F16_DEV && fluid->storageScaleExp() != scale_exp_))
     fluid->setStorage(storage_, scale_exp_);
@@ -251,7 +251,7 @@ void Batch::rebuild() {
     // same rule as the single-lattice path: stream the stores past the
     // 256 MB Infinity Cache once the state no longer fits in it
     launch_.nt = fbytes > (size_t)192 << 20 ? 1 : 0;
-    launch_.run = launch_.nt ? 4 : 0;
+    launch_.run = launch_.nt ? 16 : 0;
     launch_.grid = 0;
     launch_.wgb = 1;
     launch_.chunk = 1;

@@ -49,7 +49,9 @@ __device__ __forceinline__ GLSM_GLOBAL T* gmem(T* p) {
 //   RUN = 0: one contiguous run per XCD -- best while the state lives in the
 //            Infinity Cache;
 //   RUN > 0: runs of RUN blocks, dealt round-robin -- keeps the DRAM access
-//            front compact for HBM-resident lattices (RUN = 4 measured best).
+//            front compact for HBM-resident lattices (RUN = 16 measured best
+//            with the 256-cell block layout: 201^3 FP32 205.1 vs 206.6 us for
+//            RUN = 4, FP16 104.3 vs 106.6 us).
 template <int RUN>
 __device__ __forceinline__ unsigned xcd_block(unsigned id, unsigned nb) {
   constexpr unsigned kXcd = 8;
@@ -220,6 +222,15 @@ __host__ __device__ __forceinline__ unsigned dist_index(unsigned i, unsigned q) 
 __host__ __device__ __forceinline__ size_t dist_cells(size_t n) {  // n rounded up to whole blocks
   return (n + kDistBlock - 1) / kDistBlock * kDistBlock;
 }
+// The stream/collide kernel's loads address a population's source by a
+// 32-bit byte offset from a base kPullBias runs before that population's
+// run (so that a bounce-back source, whose run can lie before q's, still has
+// a non-negative offset).  A buffer of `plane` cells takes these offsets
+// when dist_fits_b32(plane, bytes per value).
+constexpr unsigned kPullBias = 38;
+inline bool dist_fits_b32(size_t plane, size_t elem) {
+  return (19ull * plane + (unsigned long long)kPullBias * kDistBlock) * elem < (1ull << 32);
+}
 size_t storage_bytes(int storage);
 // device copy of `bytes` (multiple of 16), 16 B per lane
 void launch_copy16(const void* a, void* b, size_t bytes, hipStream_t s);
@@ -237,7 +248,7 @@ void launch_lbm_step(int storage, const LatticeGeom& g, const LbmParams& p, cons
                      size_t plane, float3* u, bool write_u_all, const SpreadLists& sl, const float3* Fl,
                      hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 // XCD run length of the single-lattice stream/collide block order
-// (-1 = automatic: 4 when the state streams from HBM, else 0); a
+// (-1 = automatic: 16 when the state streams from HBM, else 0); a
 // measurement switch, process-wide
 void set_lattice_xcd_run(int run);
 void launch_lbm_init(int storage, const LatticeGeom& g, const LbmParams& p, void* f, float3* u, float* c1,

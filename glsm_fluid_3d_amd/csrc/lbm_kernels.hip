@@ -162,12 +162,15 @@ __device__ inline void decode_cell(const LatticeGeom& g, int i, int& x, int& y, 
 }
 
 // Pull-source addressing of one cell: for each axis, the flat offset to the
-// upstream neighbour of a +1 / -1 velocity and whether it exists.
-struct PullAddr {
+// upstream neighbour of a +1 / -1 velocity and whether it exists.  Offsets
+// are in units of E bytes: E = 1 gives cell offsets, E = sizeof(T) byte
+// offsets (src_rel).
+template <unsigned E = 1>
+struct PullAddrT {
   int oxp, oxm, oyp, oym, ozp, ozm;
   bool kxp, kxm, kyp, kym, kzp, kzm;
-  __device__ PullAddr(const LatticeGeom& g, int x, int y, int z) {
-    const int sx = 1, sy = g.Nx, sz = g.Nx * g.Ny;
+  __device__ PullAddrT(const LatticeGeom& g, int x, int y, int z) {
+    const int sx = (int)E, sy = g.Nx * (int)E, sz = g.Nx * g.Ny * (int)E;
     const bool xl = x > 0, xh = x < g.Nx - 1, yl = y > 0, yh = y < g.Ny - 1, zl = z > g.zlo, zh = z < g.zhi;
     kxp = xl || g.perX; oxp = xl ? -sx : (g.Nx - 1) * sx;
     kxm = xh || g.perX; oxm = xh ? sx : -(g.Nx - 1) * sx;
@@ -176,8 +179,39 @@ struct PullAddr {
     kzp = zl || g.perZ; ozp = zl ? -sz : (g.Nz - 1) * sz;
     kzm = zh || g.perZ; ozm = zh ? sz : -(g.Nz - 1) * sz;
   }
+  // Byte form (E = sizeof(T)): the source of population q arriving at the
+  // cell whose offset in bytes is ib = i * E and whose population 0 sits at
+  // byte eib = dist_index(i, 0) * E, as a byte offset from pull_base(f, q)
+  // (kPullBias runs before population q's run; every offset is then
+  // non-negative, also for a bounce-back to a population whose run lies
+  // before q's in the buffer's first block).  Per source: one add, one
+  // shift and one multiply-add; the run offset is uniform and goes to the
+  // scalar base.
+  template <int q>
+  __device__ __forceinline__ unsigned src_rel(unsigned ib, unsigned eib) const {
+    static_assert(E == 2 || E == 4, "byte form for 16- and 32-bit storage");
+    constexpr unsigned SH = E == 2 ? 9 : 10;  // bytes of one run: kDistBlock * E = 2^SH
+    static_assert(kDistBlock * E == (1u << SH), "256-value runs");
+    bool ok = true;
+    int off = 0;
+    if (CX[q] > 0) { ok = ok && kxp; off += oxp; }
+    if (CX[q] < 0) { ok = ok && kxm; off += oxm; }
+    if (CY[q] > 0) { ok = ok && kyp; off += oyp; }
+    if (CY[q] < 0) { ok = ok && kym; off += oym; }
+    if (CZ[q] > 0) { ok = ok && kzp; off += ozp; }
+    if (CZ[q] < 0) { ok = ok && kzm; off += ozm; }
+    constexpr unsigned bias = kPullBias << SH;
+    if (q == 0) return eib + bias;
+    // jbb = (i + off) * E + 2 runs: jbb >> SH is then the source's block
+    // + 2, and the multiply-add lands 2 * 18 + 2 = kPullBias runs past
+    // dist_index(i + off, 0) * E
+    static_assert(kPullBias == 2 * 18 + 2, "bias of the multiply-add form");
+    const unsigned jbb = (unsigned)((int)ib + off) + (2u << SH);
+    const unsigned ejb = jbb + (jbb >> SH) * (18u << SH);
+    return ok ? ejb : eib + (unsigned)((OPP[q] - q + (int)kPullBias) * (int)(1u << SH));
+  }
   // distribution element (dist_index) of the source of population q
-  // arriving at cell i
+  // arriving at cell i (E = 1)
   template <int q>
   __device__ __forceinline__ unsigned src(unsigned i) const {
     bool ok = true;
@@ -211,6 +245,8 @@ struct PullAddr {
   }
 };
 
+using PullAddr = PullAddrT<1>;
+
 template <int q>
 struct Unroll {
   template <class F>
@@ -236,6 +272,56 @@ __device__ __forceinline__ T* elem_at(T* f, unsigned idx) {
   if (B32) return reinterpret_cast<T*>(reinterpret_cast<C*>(f) + idx * (unsigned)sizeof(T));
   return f + idx;
 }
+// Loads of the stream/collide kernel: population q's run start minus
+// kPullBias runs, as a uniform address (formed as an integer: it may lie
+// before the buffer; src_rel's offsets from it are at least kPullBias runs,
+// so every address loaded is inside the buffer).
+// A pointer every lane of the wavefront holds, as a scalar value (the
+// batched kernels of several blocks per work-group pick the member per
+// group of four wavefronts, which the compiler cannot see to be uniform).
+template <class T>
+__device__ __forceinline__ T* wave_uniform(T* p) {
+  const uint64_t v = reinterpret_cast<uintptr_t>(p);
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v), hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return reinterpret_cast<T*>(((uint64_t)hi << 32) | lo);
+}
+
+template <class T>
+__device__ __forceinline__ const GLSM_GLOBAL T* pull_at(const GLSM_GLOBAL T* f, int q, unsigned rel) {
+  uintptr_t base =
+      reinterpret_cast<uintptr_t>(f) + (uintptr_t)(intptr_t)((q - (int)kPullBias) * (int)(kDistBlock * sizeof(T)));
+  // kept a scalar-register value the compiler cannot fold into the per-lane
+  // offset: the load is then scalar base + 32-bit lane offset, not a 64-bit
+  // lane address built with four vector instructions
+  asm("" : "+s"(base));
+  return reinterpret_cast<const GLSM_GLOBAL T*>(base + rel);
+}
+
+// Population q of a cell in the block layout, from the start of q's run of
+// the buffer's first block (f + q * kDistBlock, a uniform address) plus the
+// cell's offset `rel`: bytes when B32, else values.
+template <bool B32, class T>
+__device__ __forceinline__ T* run_at(T* f, unsigned q, unsigned rel) {
+  using C = typename std::conditional<std::is_const<T>::value, const GLSM_GLOBAL char, GLSM_GLOBAL char>::type;
+  if (B32) return reinterpret_cast<T*>(reinterpret_cast<C*>(f + q * kDistBlock) + rel);
+  return f + q * kDistBlock + rel;
+}
+
+// c_q . u: the sum of the nonzero components (c has components 0 and +-1,
+// at most two of them nonzero), the value cx*ux + cy*uy + cz*uz rounds to
+// (up to the sign of a zero result)
+template <int q>
+__device__ __forceinline__ float c_dot(float ux, float uy, float uz) {
+  constexpr int cx = CX[q], cy = CY[q], cz = CZ[q];
+  const float tx = cx > 0 ? ux : -ux, ty = cy > 0 ? uy : -uy, tz = cz > 0 ? uz : -uz;
+  if (cx != 0 && cy != 0) return tx + ty;
+  if (cx != 0 && cz != 0) return tx + tz;
+  if (cy != 0 && cz != 0) return ty + tz;
+  if (cx != 0) return tx;
+  if (cy != 0) return ty;
+  if (cz != 0) return tz;
+  return 0.f;
+}
 
 // BGK + Guo on one cell.  v[] holds the stored units (f, d = f - w, or d
 // scaled by S = 2^scale_exp); the same units are written back.  The scale
@@ -252,8 +338,8 @@ __device__ __forceinline__ T* elem_at(T* f, unsigned idx) {
 // accumulating 19 roundings per sub-step.
 template <int S, bool FORCED, bool NT, bool B32>
 __device__ __forceinline__ void collide_store(const float (&v)[19], float Fx, float Fy, float Fz, const LbmParams& p,
-                                              GLSM_GLOBAL typename Store<S>::T* __restrict__ fout, unsigned i,
-                                              GLSM_GLOBAL Vec3* __restrict__ u, bool store_u) {
+                                              GLSM_GLOBAL typename Store<S>::T* __restrict__ fout, unsigned own,
+                                              unsigned i, GLSM_GLOBAL Vec3* __restrict__ u, bool store_u) {
   constexpr bool DEV = Store<S>::dev;
   constexpr bool MOM64 = S == F16_DEV || S == BF16_DEV;
   constexpr bool SCALED = Store<S>::scaled;
@@ -261,14 +347,16 @@ __device__ __forceinline__ void collide_store(const float (&v)[19], float Fx, fl
   float s, rho, mx, my, mz;  // (stored units) mass deviation or mass, momentum incl. F/2; rho itself
   double sd = 0.0;           // MOM64: exact mass deviation (stored units)
   if (MOM64) {
+    // exact sums, so any grouping gives the same values: the six one-sided
+    // momentum sums are shared with the mass sum
+    double d[19];
 #pragma unroll
-    for (int q = 0; q < 19; ++q) sd += (double)v[q];
-    double mxd = ((double)v[1] - v[2]) + ((double)v[7] - v[8]) + ((double)v[10] - v[9]) +
-                 ((double)v[11] - v[12]) + ((double)v[14] - v[13]);
-    double myd = ((double)v[3] - v[4]) + ((double)v[7] - v[10]) + ((double)v[8] - v[9]) +
-                 ((double)v[15] - v[17]) + ((double)v[16] - v[18]);
-    double mzd = ((double)v[5] - v[6]) + ((double)v[11] - v[14]) + ((double)v[12] - v[13]) +
-                 ((double)v[15] - v[16]) + ((double)v[18] - v[17]);
+    for (int q = 0; q < 19; ++q) d[q] = (double)v[q];
+    const double px = (d[1] + d[7]) + (d[10] + d[11]) + d[14], nx = (d[2] + d[8]) + (d[9] + d[12]) + d[13];
+    const double py = (d[3] + d[7]) + (d[8] + d[15]) + d[16], ny = (d[4] + d[9]) + (d[10] + d[17]) + d[18];
+    const double pz = (d[5] + d[11]) + (d[12] + d[15]) + d[18], nz = (d[6] + d[13]) + (d[14] + d[16]) + d[17];
+    sd = ((d[0] + px) + nx) + (((d[3] + d[4]) + (d[5] + d[6])) + ((d[15] + d[16]) + (d[17] + d[18])));
+    double mxd = px - nx, myd = py - ny, mzd = pz - nz;
     if (FORCED) {
       const double h = 0.5 * (double)sc;  // F/2 in stored units (exact)
       mxd += h * (double)Fx;
@@ -310,7 +398,7 @@ __device__ __forceinline__ void collide_store(const float (&v)[19], float Fx, fl
     if (MOM64 && q == 0) return;  // stored last
     constexpr float cx = (float)CX[q], cy = (float)CY[q], cz = (float)CZ[q];
     const float w = wq(p, q);
-    const float cu = cx * ux + cy * uy + cz * uz;  // c has components 0, +-1
+    const float cu = c_dot<q>(ux, uy, uz);
     const float poly = fmaf(cu, fmaf(4.5f, cu, 3.0f), -uu15);
     // feq = w rho (1 + poly); deviation form: feq - w = w (s + rho poly);
     // out = v + omega (feq - v)
@@ -321,10 +409,10 @@ __device__ __forceinline__ void collide_store(const float (&v)[19], float Fx, fl
       const float ciF = cx * Fx + cy * Fy + cz * Fz;
       out = fmaf(SCALED ? w * kF * sc : w * kF, fmaf(9.0f * cu, ciF, 3.0f * (ciF - uF)), out);
     }
-    const float st = Store<S>::template st<NT>(elem_at<B32>(fout, dist_index(i, q)), out);
+    const float st = Store<S>::template st<NT>(run_at<B32>(fout, q, own), out);
     if (MOM64) kept += st;
   });
-  if (MOM64) Store<S>::template st<NT>(elem_at<B32>(fout, dist_index(i, 0)), (float)(sd - (double)kept));
+  if (MOM64) Store<S>::template st<NT>(run_at<B32>(fout, 0, own), (float)(sd - (double)kept));
 }
 
 // Every access goes through GLSM_GLOBAL pointers: the batched kernels take
@@ -342,13 +430,29 @@ __device__ __forceinline__ void lbm_block(const LatticeGeom& g, const LbmParams&
   if (i >= (unsigned)g.N) return;
   int x, y, z;
   decode_cell(g, (int)i, x, y, z);
-  const PullAddr a(g, x, y, z);
-  const GLSM_GLOBAL typename Store<S>::T* gin = gmem(fin);
+  using T = typename Store<S>::T;
+  constexpr unsigned E = sizeof(T);
+  const GLSM_GLOBAL T* gin = gmem(wave_uniform(fin));
   float v[19];
-  Unroll<19>::run([&](auto qc) {
-    constexpr int q = decltype(qc)::value;
-    v[q] = Store<S>::ld(elem_at<B32>(gin, a.src<q>(i)));
-  });
+  // own: the cell's population-0 offset, bytes (B32) or values
+  unsigned own;
+  if constexpr (B32) {
+    constexpr unsigned SH = E == 2 ? 9 : 10;
+    const unsigned ib = i * E;
+    own = ib + (ib >> SH) * (18u << SH);  // dist_index(i, 0) * E
+    const PullAddrT<E> a(g, x, y, z);
+    Unroll<19>::run([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      v[q] = Store<S>::ld(pull_at(gin, q, a.template src_rel<q>(ib, own)));
+    });
+  } else {
+    own = dist_index(i, 0);
+    const PullAddr a(g, x, y, z);
+    Unroll<19>::run([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      v[q] = Store<S>::ld(gin + a.src<q>(i));
+    });
+  }
   bool store_u = write_u_all;
   GLSM_GLOBAL Vec3* gu = gmem(reinterpret_cast<Vec3*>(u));
   if (flagged || const_force) {
@@ -389,9 +493,9 @@ __device__ __forceinline__ void lbm_block(const LatticeGeom& g, const LbmParams&
           }
       }
     }
-    collide_store<S, true, NT, B32>(v, p.Fc.x + Fx, p.Fc.y + Fy, p.Fc.z + Fz, p, gmem(fout), i, gu, store_u);
+    collide_store<S, true, NT, B32>(v, p.Fc.x + Fx, p.Fc.y + Fy, p.Fc.z + Fz, p, gmem(fout), own, i, gu, store_u);
   } else {
-    collide_store<S, false, NT, B32>(v, 0.f, 0.f, 0.f, p, gmem(fout), i, gu, store_u);
+    collide_store<S, false, NT, B32>(v, 0.f, 0.f, 0.f, p, gmem(fout), own, i, gu, store_u);
   }
 }
 
@@ -677,7 +781,7 @@ void launch_maybe_timed(K kernel, int grid, hipStream_t s, hipEvent_t ev0, hipEv
 }
 
 // a distribution buffer whose every byte offset fits in 32 bits
-bool fits_b32(size_t plane, size_t elem) { return 19ull * plane * elem < (1ull << 32); }
+bool fits_b32(size_t plane, size_t elem) { return dist_fits_b32(plane, elem); }
 
 std::atomic<int> g_lattice_run{-1};  // forced XCD run length of the single-lattice kernel, -1 = automatic
 
@@ -697,13 +801,13 @@ void step_impl(const LatticeGeom& g, const LbmParams& p, const void* fin, void* 
   auto go = [&](auto kernel) {
     launch_maybe_timed(kernel, blocks_for(g.N), s, ev0, ev1, g, p, in, out, u, wu, sl, Fl, const_force);
   };
-  // XCD run length: 4-block runs once the state streams from HBM, one
+  // XCD run length: 16-block runs once the state streams from HBM, one
   // contiguous run per XCD while it stays in the Infinity Cache, unless
   // set_lattice_xcd_run() forced another
   const int forced = g_lattice_run.load(std::memory_order_relaxed);
-  const int run = forced >= 0 ? forced : (nt ? 4 : 0);
+  const int run = forced >= 0 ? forced : (nt ? 16 : 0);
   if (!b32) {
-    nt ? go(k_lbm_step<S, true, false, 4>) : go(k_lbm_step<S, false, false, 0>);
+    nt ? go(k_lbm_step<S, true, false, 16>) : go(k_lbm_step<S, false, false, 0>);
     return;
   }
   switch (run * 2 + (nt ? 1 : 0)) {
