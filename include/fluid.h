@@ -119,8 +119,12 @@ class Fluid {
   float* h_c1;
   FluidParams* h_fp;
   // device data
-  void* d_f;      // post-collision populations (storage representation), current
-  void* d_fnext;  // post-collision populations, next
+  // post-collision populations (storage representation) in the 256-cell
+  // block layout: population q of cell i at (i / 256) * 19 * 256 + q * 256 +
+  // i % 256 (the reference's d_f holds one plane of N values per
+  // population; readPopulations / writePopulations convert)
+  void* d_f;      // current
+  void* d_fnext;  // next
   float3* d_u;
   float* d_c1;
   float* d_c2;
