@@ -121,6 +121,18 @@ struct Store<F16_DEV> {
     put<T, NT>(p, h);
     return (float)h;
   }
+  // stores v as st() does and returns acc + the value kept, as one
+  // mixed-precision multiply-add (h * 1.0 + acc, rounded once: the same
+  // value as converting h to FP32 and adding)
+  template <bool NT, class P>
+  __device__ static float st_add(P p, float v, float acc) {
+    asm("" : "+v"(v));
+    const _Float16 h = (_Float16)v;
+    put<T, NT>(p, h);
+    float r;
+    asm("v_fma_mix_f32 %0, %1, 1.0, %2 op_sel_hi:[1,0,0]" : "=v"(r) : "v"(h), "v"(acc));
+    return r;
+  }
 };
 template <>
 struct Store<BF16_DEV> {
@@ -136,6 +148,10 @@ struct Store<BF16_DEV> {
     const uint32_t r = (b + 0x7FFFu + ((b >> 16) & 1u)) >> 16;  // round to nearest even (finite v)
     put<T, NT>(p, (uint16_t)r);
     return __uint_as_float(r << 16);
+  }
+  template <bool NT, class P>
+  __device__ static float st_add(P p, float v, float acc) {
+    return acc + st<NT>(p, v);
   }
 };
 // stored units <-> the population (or deviation) itself
@@ -414,8 +430,10 @@ __device__ __forceinline__ void collide_store(const float (&v)[19], float Fx, fl
       const float ciF = c_dot<q>(Fx, Fy, Fz);
       out = fmaf(SCALED ? w * kF * sc : w * kF, fmaf(9.0f * cu, ciF, 3.0f * (ciF - uF)), out);
     }
-    const float st = Store<S>::template st<NT>(run_at<B32>(fout, q, own), out);
-    if (MOM64) kept = fmaf(st, 1.0f, kept);  // an add; takes a 16-bit st as a mixed-precision multiply-add
+    if constexpr (MOM64)
+      kept = Store<S>::template st_add<NT>(run_at<B32>(fout, q, own), out, kept);
+    else
+      Store<S>::template st<NT>(run_at<B32>(fout, q, own), out);
   });
   if (MOM64) Store<S>::template st<NT>(run_at<B32>(fout, 0, own), (float)(sd - (double)kept));
 }
