@@ -292,6 +292,8 @@ def main():
     ap.add_argument("--interp-mode", type=int, default=0, choices=[0, 1, 2, 3],
                     help="IBM interpolation form: 0 default (staged), 1 wavefront per marker, 2 thread per marker, "
                          "3 staged in LDS")
+    ap.add_argument("--lds-reserve", type=int, default=-1,
+                    help="LDS bytes reserved per stream/collide work-group (caps work-groups per CU); -1 automatic")
     ap.add_argument("--xcd-run", type=int, default=-1, choices=[-1, 0, 4, 16, 64],
                     help="XCD run length of the single-lattice stream/collide block order (-1 automatic)")
     ap.add_argument("--workload", default="all",
@@ -311,6 +313,7 @@ def main():
     G.init(0)
     G.set_interp_mode(args.interp_mode)
     G.set_lattice_xcd_run(args.xcd_run)
+    G.set_lbm_lds_reserve(args.lds_reserve)
     if args.workload == "copy":
         nbytes = 2 * 19 * 4 * 8120601 // 2  # one 201^3 FP32 distribution buffer
         print(json.dumps({"workload": "plain 16-B copy", "bytes": nbytes,

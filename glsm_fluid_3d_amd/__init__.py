@@ -91,6 +91,7 @@ SIGNATURES = {
     "glsm_set_interp_mode": (_i, [_i]),
     "glsm_set_repulsion_cull": (_i, [_i]),
     "glsm_set_lattice_xcd_run": (_i, [_i]),
+    "glsm_set_lbm_lds_reserve": (_i, [_i]),
     "glsm_batch_create": (_i, [_i, _i, C.POINTER(_vp)]),
     "glsm_batch_destroy": (_i, [_vp]),
     "glsm_batch_add": (_i, [_vp, _vp, _vp, _i, _i, C.c_char_p]),
@@ -99,8 +100,8 @@ SIGNATURES = {
     "glsm_batch_iteration": (_i, [_vp, _i]),
     "glsm_batch_step": (_i, [_vp, _i]),
     "glsm_batch_use_graph": (_i, [_vp, _i]),
-    "glsm_batch_set_launch": (_i, [_vp, _i, _i, _i, _i, _i]),
-    "glsm_batch_get_launch": (_i, [_vp] + [C.POINTER(_i)] * 5),
+    "glsm_batch_set_launch": (_i, [_vp, _i, _i, _i, _i, _i, _i]),
+    "glsm_batch_get_launch": (_i, [_vp] + [C.POINTER(_i)] * 6),
     "glsm_batch_autotune": (_i, [_vp, _i, C.POINTER(_d)]),
     "glsm_batch_enable_lbm_timing": (_i, [_vp, _i]),
     "glsm_batch_lbm_timing": (_i, [_vp, C.POINTER(_d), C.POINTER(_i)]),
@@ -170,6 +171,12 @@ def set_interp_mode(mode: int):
     1 = wavefront per marker, 2 = thread per marker, 3 = products staged in
     LDS (eight threads per marker)."""
     _check(load().glsm_set_interp_mode(int(mode)), "glsm_set_interp_mode")
+
+
+def set_lbm_lds_reserve(nbytes: int):
+    """LDS reserved per stream/collide work-group (caps the work-groups per
+    CU): -1 automatic (default), else 0..65536 bytes; results are identical."""
+    _check(load().glsm_set_lbm_lds_reserve(int(nbytes)), "glsm_set_lbm_lds_reserve")
 
 
 def set_lattice_xcd_run(run: int):
@@ -485,16 +492,16 @@ class Batch:
         self.output = output
         _check(load().glsm_batch_set_output(self.h, output.h if output else None), "batch set_output")
 
-    def set_launch(self, nt, run, grid, wgb=1, chunk=1):
+    def set_launch(self, nt, run, grid, wgb=1, chunk=1, lds=0):
         """Launch shape of the batched stream/collide kernel (see
         glsm_batch_set_launch)."""
-        _check(load().glsm_batch_set_launch(self.h, int(nt), int(run), int(grid), int(wgb), int(chunk)),
+        _check(load().glsm_batch_set_launch(self.h, int(nt), int(run), int(grid), int(wgb), int(chunk), int(lds)),
                "batch set_launch")
 
     def launch(self):
-        v = [C.c_int() for _ in range(5)]
+        v = [C.c_int() for _ in range(6)]
         _check(load().glsm_batch_get_launch(self.h, *[C.byref(x) for x in v]), "batch launch")
-        return dict(zip(("nt", "run", "grid", "wgb", "chunk"), (x.value for x in v)))
+        return dict(zip(("nt", "run", "grid", "wgb", "chunk", "lds"), (x.value for x in v)))
 
     def autotune(self, reps=5, cache=None):
         """Time every launch shape on the current state and keep the fastest;

@@ -62,6 +62,7 @@ glsm::LbmLaunch to_internal(const Batch::Launch& l) {
   s.grid = l.grid;
   s.wgb = l.wgb;
   s.chunk = l.chunk;
+  s.lds = l.lds;
   return s;
 }
 }  // namespace
@@ -142,6 +143,9 @@ void Batch::setLaunch(const Launch& l) {
   if (l.chunk < 1 || l.chunk > 64) throw std::invalid_argument("Batch::setLaunch: chunk must be in [1, 64]");
   if ((l.wgb != 1 || l.chunk != 1) && (l.grid != 0 || l.run == 64))
     throw std::invalid_argument("Batch::setLaunch: work units of several blocks take grid 0 and run 0, 4 or 16");
+  if (l.lds < 0 || l.lds > 65536) throw std::invalid_argument("Batch::setLaunch: lds must be in [0, 65536]");
+  if (l.lds != 0 && (l.wgb != 1 || l.chunk != 1))
+    throw std::invalid_argument("Batch::setLaunch: an LDS reserve takes one block per work-group");
   launch_ = l;
   launch_set_ = true;
   dropGraphs();
@@ -255,6 +259,9 @@ void Batch::rebuild() {
     launch_.grid = 0;
     launch_.wgb = 1;
     launch_.chunk = 1;
+    // 4-byte storage streamed from HBM: 5 work-groups per CU (measured on
+    // the 64-instance FP32 batch; the autotuner also times 4 and 3)
+    launch_.lds = launch_.nt && glsm::storage_bytes(storage_) == 4 ? 32 * 1024 : 0;
   }
   GLSM_HIP(hipStreamSynchronize(s));
   dirty_ = false;
@@ -381,8 +388,10 @@ Batch::Launch Batch::autotune(int reps, double* best_ms) {
   }
   std::vector<Launch> cand;
   for (int nt = 0; nt < 2; ++nt)
-    for (int run : {0, 4, 16, 64})
-      for (int g : {0, 2, 4, 8, 16}) cand.push_back(Launch{nt, run, g ? g * cus : 0, 1, 1});
+    for (int run : {0, 4, 16, 64}) {
+      for (int g : {0, 2, 4, 8, 16}) cand.push_back(Launch{nt, run, g ? g * cus : 0, 1, 1, 0});
+      for (int kb : {32, 40, 52}) cand.push_back(Launch{nt, run, 0, 1, 1, kb * 1024});
+    }
   for (int nt = 0; nt < 2; ++nt)
     for (int run : {0, 4, 16}) {
       for (int wgb : {2, 4}) cand.push_back(Launch{nt, run, 0, wgb, 1});

@@ -746,21 +746,22 @@ int glsm_batch_use_graph(glsm_batch* b, int on) {
     return GLSM_OK;
   });
 }
-int glsm_batch_set_launch(glsm_batch* b, int nt, int run, int grid, int wgb, int chunk) {
+int glsm_batch_set_launch(glsm_batch* b, int nt, int run, int grid, int wgb, int chunk, int lds) {
   return guarded([&] {
-    B(b)->setLaunch(Batch::Launch{nt, run, grid, wgb, chunk});
+    B(b)->setLaunch(Batch::Launch{nt, run, grid, wgb, chunk, lds});
     return GLSM_OK;
   });
 }
-int glsm_batch_get_launch(glsm_batch* b, int* nt, int* run, int* grid, int* wgb, int* chunk) {
+int glsm_batch_get_launch(glsm_batch* b, int* nt, int* run, int* grid, int* wgb, int* chunk, int* lds) {
   return guarded([&] {
-    REQUIRE(nt && run && grid && wgb && chunk, "null output pointer");
+    REQUIRE(nt && run && grid && wgb && chunk && lds, "null output pointer");
     const Batch::Launch l = B(b)->launch();
     *nt = l.nt;
     *run = l.run;
     *grid = l.grid;
     *wgb = l.wgb;
     *chunk = l.chunk;
+    *lds = l.lds;
     return GLSM_OK;
   });
 }
@@ -864,6 +865,12 @@ int glsm_device_memory(size_t* free_bytes, size_t* total_bytes) {
   });
 }
 
+int glsm_set_lbm_lds_reserve(int bytes) {
+  return guarded([&] {
+    glsm::set_lbm_lds_reserve(bytes);
+    return GLSM_OK;
+  });
+}
 int glsm_set_lattice_xcd_run(int run) {
   return guarded([&] {
     glsm::set_lattice_xcd_run(run);

@@ -251,6 +251,13 @@ void launch_lbm_step(int storage, const LatticeGeom& g, const LbmParams& p, cons
 // (-1 = automatic: 16 when the state streams from HBM, else 0); a
 // measurement switch, process-wide
 void set_lattice_xcd_run(int run);
+// LDS reserved per stream/collide work-group (single-lattice and batched,
+// one work-group per block), which caps the work-groups resident per CU:
+// -1 = automatic (single lattice: kLatticeLdsReserve for 4-byte storage
+// streamed from HBM, else 0; batch: its launch shape's), else 0..65536
+// bytes for every launch; a measurement switch, process-wide
+void set_lbm_lds_reserve(int bytes);
+constexpr int kLatticeLdsReserve = 52 * 1024;  // 3 work-groups per CU
 void launch_lbm_init(int storage, const LatticeGeom& g, const LbmParams& p, void* f, float3* u, float* c1,
                      float* c2, hipStream_t s);
 // stored state -> FP32 full populations (pull = pre-collision of the next
@@ -349,13 +356,16 @@ void launch_scalar_batch(const BatchTables& t, int cpar, hipStream_t s);
 // number of work-groups (0 = one per work unit; fewer = persistent
 // work-groups looping over the units), 256-cell blocks stepped together by
 // one work-group (wgb: 256, 512 or 1024 threads) and consecutive rounds of
-// blocks per work-group (chunk: cells per thread).
+// blocks per work-group (chunk: cells per thread).  lds: bytes of LDS
+// reserved per work-group of a one-block-per-work-group shape, which caps
+// the work-groups resident per CU (160 KB / lds).
 struct LbmLaunch {
   int nt = 0;
   int run = 0;
   int grid = 0;
   int wgb = 1;
   int chunk = 1;
+  int lds = 0;
 };
 // one sub-step of every member: f[sel] -> f[sel ^ 1]
 void launch_lbm_step_batch(int storage, const BatchTables& t, int sel, bool write_u_all, const LbmLaunch& shape,

@@ -795,12 +795,30 @@ inline int blocks_for(long n) { return (int)((n + kThreads - 1) / kThreads); }
 // A timed launch passes start/stop events that the dispatch itself
 // timestamps at kernel start and end (hipExtLaunchKernelGGL), so the sample
 // is the kernel's own duration, without launch latency.
+// LDS reserved per stream/collide work-group, which caps the work-groups
+// resident per CU (160 KB of LDS per CU): -1 = automatic, else bytes.
+// Fewer resident work-groups keep fewer DRAM streams open: with FP32 storage
+// streamed from HBM the 201^3 step ran 3.5% faster at 3 work-groups per CU
+// than at 8 (the register limit), and a 64-instance FP32 batch 6% faster at
+// 5 (its autotuner picks); 16-bit storage and cache-resident lattices need
+// the full occupancy to cover their arithmetic.
+std::atomic<int> g_lds_reserve{-1};
+size_t lds_reserve_lattice(size_t elem, bool nt) {
+  const int v = g_lds_reserve.load(std::memory_order_relaxed);
+  if (v >= 0) return (size_t)v;
+  return elem == 4 && nt ? (size_t)kLatticeLdsReserve : 0;
+}
+size_t lds_reserve_batch(const LbmLaunch& sh) {
+  const int v = g_lds_reserve.load(std::memory_order_relaxed);
+  return v >= 0 ? (size_t)v : (size_t)sh.lds;
+}
+
 template <class K, class... A>
-void launch_maybe_timed(K kernel, int grid, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1, A... args) {
+void launch_maybe_timed(K kernel, int grid, size_t lds, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1, A... args) {
   if (ev0)
-    hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(kThreads), 0, s, ev0, ev1, 0, args...);
+    hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(kThreads), lds, s, ev0, ev1, 0, args...);
   else
-    kernel<<<grid, kThreads, 0, s>>>(args...);
+    kernel<<<grid, kThreads, lds, s>>>(args...);
 }
 
 // a distribution buffer whose every byte offset fits in 32 bits
@@ -822,7 +840,8 @@ void step_impl(const LatticeGeom& g, const LbmParams& p, const void* fin, void* 
   const int wu = write_u_all ? 1 : 0;
   const bool b32 = fits_b32(plane, sizeof(T));
   auto go = [&](auto kernel) {
-    launch_maybe_timed(kernel, blocks_for(g.N), s, ev0, ev1, g, p, in, out, u, wu, sl, Fl, const_force);
+    launch_maybe_timed(kernel, blocks_for(g.N), lds_reserve_lattice(sizeof(T), nt), s, ev0, ev1, g, p, in, out, u,
+                       wu, sl, Fl, const_force);
   };
   // XCD run length: 16-block runs once the state streams from HBM, one
   // contiguous run per XCD while it stays in the Infinity Cache, unless
@@ -849,6 +868,11 @@ void step_impl(const LatticeGeom& g, const LbmParams& p, const void* fin, void* 
 }  // namespace
 
 size_t storage_bytes(int storage) { return storage == F16_DEV || storage == BF16_DEV ? 2 : 4; }
+
+void set_lbm_lds_reserve(int bytes) {
+  if (bytes < -1 || bytes > 64 * 1024) throw std::invalid_argument("LDS reserve must be -1 (automatic) or 0..65536 bytes");
+  g_lds_reserve = bytes;
+}
 
 void set_lattice_xcd_run(int run) {
   if (run != -1 && run != 0 && run != 4 && run != 16 && run != 64)
@@ -922,7 +946,8 @@ void step_batch_impl(const BatchTables& t, int sel, bool write_u_all, const LbmL
     return;
   }
   const int grid = sh.grid > 0 && sh.grid < t.nblk ? sh.grid : t.nblk;
-  auto go = [&](auto kernel) { launch_maybe_timed(kernel, grid, s, ev0, ev1, t, sel, wu); };
+  const size_t lds = lds_reserve_batch(sh);
+  auto go = [&](auto kernel) { launch_maybe_timed(kernel, grid, lds, s, ev0, ev1, t, sel, wu); };
   switch (sh.run * 2 + (sh.nt ? 1 : 0)) {
     case 0: go(kb_lbm_step<S, false, 0>); break;
     case 1: go(kb_lbm_step<S, true, 0>); break;

@@ -68,6 +68,8 @@ class Batch {
     int grid;       // work-groups; 0 = one per work unit, else persistent
     int wgb = 1;    // 256-cell blocks stepped at once by a work-group: 1, 2, 4 (256-1024 threads)
     int chunk = 1;  // rounds of blocks per work-group (cells per thread), 1..64
+    int lds = 0;    // LDS bytes reserved per work-group (0..65536; one block per work-group only):
+                    // caps the work-groups resident per CU at 160 KB / lds
   };
   Launch launch() const;
   void setLaunch(const Launch& l);
@@ -75,9 +77,11 @@ class Batch {
   // not changed; each shape's time is the median of three trials of `reps`
   // launches) and keeps the fastest; returns it and its mean launch time.
   // Candidates: NT stores x XCD run length x grid (one work-group per unit,
-  // or 2-16 per CU persistent) with one block per work-group, and work-group
-  // sizes of 512 / 1024 threads and 2-8 rounds of blocks per work-group
-  // (one work-group per unit).
+  // or 2-16 per CU persistent) with one block per work-group, the
+  // one-work-group-per-block shapes also with 32, 40 and 52 KB of LDS
+  // reserved (5, 4 and 3 work-groups per CU), and work-group sizes of 512 /
+  // 1024 threads and 2-8 rounds of blocks per work-group (one work-group
+  // per unit).
   Launch autotune(int reps, double* best_ms);
 
   // Mean duration of the batched stream/collide kernel (start/stop events

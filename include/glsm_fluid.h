@@ -227,6 +227,12 @@ int glsm_set_interp_mode(int mode);
    or 64.  Results are identical.  Process-wide. */
 int glsm_set_lattice_xcd_run(int run);
 
+/* Extension (measurement): LDS reserved per work-group of the
+   stream/collide kernels (single-lattice and batched), which caps the
+   work-groups resident per CU at 160 KB / reserve: -1 = automatic, else
+   0..65536 bytes.  Results are identical.  Process-wide. */
+int glsm_set_lbm_lds_reserve(int bytes);
+
 /* Extension (measurement and tests): from `min_markers` markers on (default
    4096; 0 = never), Coupler::applyGelRepulsion skips, for each marker, the
    gels whose markers' bounding box lies beyond the cut-off.  The sum is
@@ -258,9 +264,11 @@ int glsm_batch_use_graph(glsm_batch* b, int on);
    (0, 4, 16 or 64), grid = work-groups (0 = one per work unit, else
    persistent), wgb = 256-cell blocks stepped at once by a work-group (1, 2
    or 4: 256, 512 or 1024 threads), chunk = rounds of consecutive blocks per
-   work-group (cells per thread, 1..64); a work unit is wgb * chunk blocks */
-int glsm_batch_set_launch(glsm_batch* b, int nt, int run, int grid, int wgb, int chunk);
-int glsm_batch_get_launch(glsm_batch* b, int* nt, int* run, int* grid, int* wgb, int* chunk);
+   work-group (cells per thread, 1..64); a work unit is wgb * chunk blocks;
+   lds = LDS bytes reserved per work-group (0..65536, one block per
+   work-group only), which caps the work-groups resident per CU */
+int glsm_batch_set_launch(glsm_batch* b, int nt, int run, int grid, int wgb, int chunk, int lds);
+int glsm_batch_get_launch(glsm_batch* b, int* nt, int* run, int* grid, int* wgb, int* chunk, int* lds);
 /* time every launch shape on the current state (unchanged) and keep the
    fastest; best_ms = its mean stream/collide launch time */
 int glsm_batch_autotune(glsm_batch* b, int reps, double* best_ms);

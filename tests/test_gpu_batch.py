@@ -97,6 +97,8 @@ def test_batch_fp16_and_every_launch_shape(glsm):
     # of up to 8 blocks straddle the members' block ranges
     shapes += [(nt, run, 0, wgb, chunk) for nt in (0, 1) for run in (0, 4, 16)
                for wgb, chunk in ((2, 1), (4, 1), (1, 3), (1, 8), (2, 2))]
+    # LDS reserved per work-group (fewer work-groups resident per CU)
+    shapes += [(nt, run, 0, 1, 1, kb * 1024) for nt in (0, 1) for run in (4, 16) for kb in (32, 52, 64)]
     for shape in shapes:
         b = G.Batch("f16")
         insts = [build(G, s, "f16") for s in SPECS]
@@ -125,6 +127,7 @@ def test_batch_autotune_keeps_state_and_results(glsm):
         before = [state(inst) for inst in insts]
         shape, ms = b.autotune(reps=2)
         assert ms > 0 and shape["run"] in (0, 4, 16, 64) and shape["wgb"] in (1, 2, 4) and 1 <= shape["chunk"] <= 8
+        assert shape["lds"] in (0, 32 * 1024, 40 * 1024, 52 * 1024)
         for k, inst in enumerate(insts):
             assert_same(state(inst), before[k], f"autotune changed member {k}")
         b.step(5)
@@ -208,6 +211,10 @@ def test_batch_rejects_bad_members(glsm):
             b.set_launch(0, 3, 0)
         with pytest.raises(G.GlsmError, match="several blocks"):
             b.set_launch(0, 4, 256, 2, 1)
+        with pytest.raises(G.GlsmError, match="lds must be in"):
+            b.set_launch(0, 4, 0, 1, 1, 70000)
+        with pytest.raises(G.GlsmError, match="LDS reserve takes one block"):
+            b.set_launch(0, 4, 0, 2, 1, 32768)
     finally:
         b.close()
         close(inst)
