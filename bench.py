@@ -94,7 +94,10 @@ def run_batch(G, storage, steps, warmup, count=64, tune=True):
            "launch": shape, "hip_graph": True,
            "roofline": roofline(cells, 2 * 19 * G.STORAGE_BYTES[storage], lbm_ms)}
     per_shape = load_traffic(f"batch64_{storage}") if count == 64 else None
-    out["roofline"]["traffic"] = (per_shape.get(f"nt{shape['nt']}_run{shape['run']}")
+    # (an LDS reserve changes the work-groups per CU, not the bytes: the 201^3
+    # FP32 launch moved 1,265.1 MB with and without one)
+    key = f"nt{shape['nt']}_run{shape['run']}"
+    out["roofline"]["traffic"] = (per_shape.get(key)
                                   if isinstance(per_shape, dict) and shape.get("grid") == 0 else None)
     out["roofline"]["kernel"] = "kb_lbm_step (batched persistent pull stream + collide)"
     out["roofline"]["samples"] = samples
