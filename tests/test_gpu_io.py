@@ -170,6 +170,14 @@ def test_snapshot_rejects_other_shapes(glsm, tmp_path):
             f4.load_state(str(tmp_path / "fluid.snap"))
         with pytest.raises(G.GlsmError, match="cannot open"):
             g.load_state(str(tmp_path / "absent.snap"))
+        # a snapshot of another format version (version 1 stored one plane of
+        # N values per population, not the block layout) is refused, not misread
+        raw = bytearray(open(tmp_path / "fluid.snap", "rb").read())
+        assert raw[:8] == b"GLSMSNAP" and int.from_bytes(raw[8:12], "little") == 2
+        raw[8:12] = (1).to_bytes(4, "little")
+        open(tmp_path / "fluid_v1.snap", "wb").write(bytes(raw))
+        with pytest.raises(G.GlsmError, match="not a snapshot file of this version"):
+            f.load_state(str(tmp_path / "fluid_v1.snap"))
     finally:
         close_all(g, f, c, g4, f4, c4)
 
