@@ -552,7 +552,7 @@ __global__ void __launch_bounds__(kThreads, 8) kb_lbm_step(BatchTables t, int se
 // The same in work units of WGB * chunk consecutive 256-cell blocks: a
 // work-group of WGB * 256 threads takes one unit, its WGB groups of four
 // waves each one block per round, for `chunk` rounds over consecutive
-// blocks (the shifted planes' edge lines of round r are read again in round
+// blocks (the shifted sources' edge lines of round r are read again in round
 // r + 1 by the same CU).  One work-group per unit, the XCD remap applied to
 // the unit index.
 template <int S, bool NT, int RUN, int WGB>

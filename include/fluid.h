@@ -5,8 +5,9 @@
 // its drivers read (Nsub, h_fp->h).
 //
 // State layout in HBM (differs from the reference on purpose):
-//   * distributions: two ping-pong buffers of 19 structure-of-arrays planes,
-//     each plane padded to a 256-byte multiple; they hold POST-collision
+//   * distributions: two ping-pong buffers in blocks of 256 cells, each
+//     block holding the 19 populations as 19 runs of 256 values (see d_f
+//     below for the index formula); they hold POST-collision
 //     populations (pull scheme: one fused stream+collide pass per sub-step),
 //     by default as FP32 deviations f - w_q (see Storage below);
 //   * d_u (float3, x fastest), d_c1/d_c2 (float) as in the reference;
