@@ -26,6 +26,13 @@ sim_peristaltic.cpp (two 40x40x6 slabs in an 81x81x61 channel with
 GEL_SIZE_X=40, pulse at iteration 500) runs the coupled step with two gels
 for 1000 iterations: its status lines and files are compared the same way.
 
+sim.cpp (two 20x20x4 gels in a 150x50x10 channel), sim_array.cpp (a 3x3
+array of 20x20x4 gels, pulse delays drawn with srand(42)/rand()) and
+sim_cilia.cpp (a 4x4 array of anchored 4x4x24 gels, the same delays) fix
+their length at compile time; each runs until its status line of iteration
+3000 and is stopped, and its status lines and the files of snapshots 0 and
+1 must match the same loop through the C ABI.
+
 The sim_single scene is then driven through the C ABI from Python with the
 driver's call sequence -- per-object methods before iteration 500,
 glsm_system_step (sim.cpp's loop body, the same calls) from there on -- and
@@ -33,9 +40,14 @@ the objects' writeFiles into a second directory.  Every .dat file must be
 byte-identical.  The driver is built only where the reference tree is
 mounted; the test fails if it is missing.
 """
+import ctypes
 import os
+import queue
+import re
 import subprocess
 import sys
+import threading
+import time
 
 import numpy as np
 import pytest
@@ -338,3 +350,164 @@ def test_sim_peristaltic_driver_prints_and_writes_the_same(glsm, tmp_path):
                  for t in (0, 1)}
     same_files(run_dir / "data", mir, gel_files | {"Velb0.dat", "Conc0.dat"},
                skip={"gel1Veln0.dat", "gel2Veln0.dat"})
+
+
+# ------------------------------------------- drivers of fixed length
+# sim.cpp, sim_array.cpp and sim_cilia.cpp fix their length at compile time
+# (`#define runstep` 150000 / 200000, sim.cpp:15, sim_array.cpp:17,
+# sim_cilia.cpp:25).  Each runs until it prints its status line of
+# iteration 3000 and is then stopped.  By then writeFiles(2000) has joined
+# the writer threads of snapshot 1 (gel.cu:543-549, fluid.cu:144-146), so
+# every file of snapshots 0 and 1 is complete; later ones may be partial
+# and are not compared.
+STOP_AT = 3000
+SNAPSHOT = re.compile(r"^(?:gel\d+(?:rn|Fn|Veln|rm|um|vm|wm)|Velb|Conc)(\d+)\.dat$")
+
+
+def run_driver_until(name, run_dir, last_iter, env_extra=None, timeout=240):
+    """Start oracle/_ref/<name> in run_dir, collect its stdout until the
+    status line of `last_iter`, then stop it; returns the lines read."""
+    exe = os.path.join(ROOT, "oracle", "_ref", name)
+    assert os.path.exists(exe), f"{exe} is missing: build it with `make -C oracle drivers`"
+    env = dict(os.environ)
+    env.update(env_extra or {})
+    print(f"running {name} until iteration {last_iter}", flush=True)
+    err = open(run_dir / "stderr.txt", "w")
+    p = subprocess.Popen([exe], cwd=str(run_dir), env=env, stdout=subprocess.PIPE, stderr=err, text=True)
+    lines, q = [], queue.Queue()
+
+    def reader():
+        for ln in p.stdout:
+            q.put(ln.rstrip("\n"))
+        q.put(None)
+
+    t = threading.Thread(target=reader, daemon=True)
+    t.start()
+    deadline = time.monotonic() + timeout
+    reached = False
+    try:
+        while True:
+            ln = q.get(timeout=max(1.0, deadline - time.monotonic()))
+            if ln is None:
+                break
+            lines.append(ln)
+            m = re.match(r"iter=\s*(\d+)\s", ln)
+            if m and int(m.group(1)) >= last_iter:
+                reached = True
+                break
+    finally:
+        if p.poll() is None:
+            p.kill()
+        p.wait(timeout=60)
+        err.close()
+    sys.stdout.write("\n".join(lines[-8:]) + "\n")
+    assert reached, f"{name} stopped before iteration {last_iter}: " + (run_dir / "stderr.txt").read_text()[-3000:]
+    return lines
+
+
+def delayed_array_sequence(G, fluid_size, gel_size, centres, delays, last_iter, anchor_z=None):
+    """The loop shared by sim.cpp:44-111, sim_array.cpp:66-133 and
+    sim_cilia.cpp:73-135 through the C ABI: gel k (id k + 1) stays quiescent
+    until its delay, when its excitation pulse fires; status lines every
+    1000 iterations; files every iteration as the drivers write them."""
+    gels = []
+    for k, (pos, d) in enumerate(zip(centres, delays)):
+        g = G.Gel(gel_size, pos, gel_type=1, gel_id=k + 1, time=0)
+        if anchor_z is not None:
+            g.set_anchor_z(anchor_z)
+        if d > 0:
+            g.reset_to_quiescent()
+        gels.append(g)
+    c = G.Coupler(gels)
+    f = G.Fluid(fluid_size, c)
+    h = fluid_size[0] / (f.grid[0] - 1)
+    lines = []
+    for it in range(last_iter + 1):
+        for g, d in zip(gels, delays):
+            if d > 0 and it == d:
+                g.fire_excitation_pulse()
+        for g, d in zip(gels, delays):
+            if d > 0 and it < d:
+                g.record_center(it)
+                continue
+            g.step_elasticity(it)
+            g.step_chemistry(it)
+            g.record_center(it)
+        c.pack_from_gels()
+        f.step_velocity(it)
+        f.step_concentration(it)
+        c.apply_gel_repulsion()
+        c.scatter_to_gels()
+        if it % 1000 == 0:
+            vg = np.float32(0.0)
+            for g in gels:
+                v = g.read("Veln")
+                vg = max(vg, np.sqrt(v[:, 0] * v[:, 0] + v[:, 1] * v[:, 1] + v[:, 2] * v[:, 2]).astype(np.float32).max())
+            lines.append("iter=%6d  Vg_max=%.5f  lbm2gel=%.2f" % (it, float(vg), np.float32(f.nsub) / np.float32(h)))
+        for g in gels:
+            g.write_files(it)
+        f.write_files(it)
+    for o in gels + [f, c]:
+        o.close()
+    return lines
+
+
+def libc_delays(n, seed=42, period=50000.0):
+    """(int)((double)rand() / RAND_MAX * T) for n gels after srand(seed), as
+    sim_array.cpp:43-51 and sim_cilia.cpp:48-53 draw them (glibc rand)."""
+    libc = ctypes.CDLL(None)
+    libc.srand(ctypes.c_uint(seed))
+    return [int(libc.rand() / 2147483647 * period) for _ in range(n)]
+
+
+def compare_first_snapshots(drv_dir, mir_dir, ngel):
+    """Files of snapshots 0 and 1 in both directories: the same names, and
+    the same bytes except gel*Veln{0,1}.dat (the drivers copy m_dVeln into
+    m_hVeln at every status line while that snapshot's writer thread may
+    still format it, gel.cu:543-553: the drivers' own race)."""
+    def first(d):
+        return {n for n in os.listdir(d) if SNAPSHOT.match(n) and int(SNAPSHOT.match(n).group(1)) <= 1}
+    expected = {f"gel{k}{n}{t}.dat" for k in range(1, ngel + 1) for n in ("rn", "Fn", "Veln", "rm", "um", "vm", "wm")
+                for t in (0, 1)} | {"Velb0.dat", "Conc0.dat"}
+    a, b = first(drv_dir), first(mir_dir)
+    assert a == b == expected, (sorted(a ^ expected), sorted(b ^ expected))
+    for name in sorted(a):
+        x, y = (drv_dir / name).read_bytes(), (mir_dir / name).read_bytes()
+        if "Veln" in name:
+            continue
+        if not x or x != y:
+            pytest.fail(f"{name}: {len(x)} bytes from the driver, {len(y)} through the C ABI, "
+                        f"first difference at byte {next((i for i, (p, q) in enumerate(zip(x, y)) if p != q), -1)}")
+
+
+@pytest.mark.parametrize("name", ["sim", "sim_array", "sim_cilia"])
+def test_fixed_length_driver_prints_and_writes_the_same(glsm, tmp_path, name):
+    if name == "sim":  # sim.cpp:26-34, GEL2_DELAY unset
+        scene = dict(fluid_size=(150, 50, 10), gel_size=(20, 20, 4), centres=[(50.0, 25.0, 5.0), (100.0, 25.0, 5.0)],
+                     delays=[0, 0])
+        env = {"GEL2_DELAY": "0"}
+    elif name == "sim_array":  # sim_array.cpp:33-78
+        scene = dict(fluid_size=(140, 140, 10), gel_size=(20, 20, 4),
+                     centres=[(float(x), float(y), 7.0) for y in (30, 70, 110) for x in (30, 70, 110)],
+                     delays=libc_delays(9))
+        env = {"RAND_SEED": "42"}
+    else:  # sim_cilia.cpp:36-84
+        scene = dict(fluid_size=(100, 100, 30), gel_size=(4, 4, 24),
+                     centres=[(float(x), float(y), 15.0) for y in (10, 30, 50, 70) for x in (10, 30, 50, 70)],
+                     delays=libc_delays(16), anchor_z=3.0)
+        env = {"RAND_SEED": "42"}
+    print(f"{name}: delays {scene['delays']}", flush=True)
+    run_dir = tmp_path / "driver"
+    run_dir.mkdir()
+    printed = [ln for ln in run_driver_until(name, run_dir, STOP_AT, env) if ln.startswith("iter=")]
+    mir = tmp_path / "mirror"
+    mir.mkdir()
+    cwd = os.getcwd()
+    os.chdir(mir)
+    try:
+        lines = delayed_array_sequence(glsm, last_iter=STOP_AT, **scene)
+    finally:
+        os.chdir(cwd)
+    print("\n".join(lines), flush=True)
+    assert printed == lines
+    compare_first_snapshots(run_dir / "data", mir, len(scene["centres"]))
