@@ -63,6 +63,23 @@ def gel_array_3x3(mod, storage=None):
     return gels, c, f
 
 
+def cilia_4x4(mod, storage=None):
+    """The 4 x 4 cilia array of sim_cilia.cpp:36-84: a 100 x 100 x 30 box
+    (201 x 201 x 61 nodes), sixteen 4 x 4 x 24-element gels at x, y in
+    {10, 30, 50, 70}, z = 15, anchored at z = 3 -- every gel stepping, as
+    once all of the driver's pulse delays have passed."""
+    gels = []
+    for k, (y, x) in enumerate((y, x) for y in (10, 30, 50, 70) for x in (10, 30, 50, 70)):
+        g = mod.Gel((4, 4, 24), (float(x), float(y), 15.0), gel_id=k + 1)
+        g.set_anchor_z(3.0)
+        gels.append(g)
+    c = mod.Coupler(gels)
+    f = mod.Fluid((100, 100, 30), c)
+    if storage:
+        f.set_storage(storage)
+    return gels, c, f
+
+
 def cube201_four_gels(mod, storage):
     from glsm_fluid_3d_amd import workloads as W
 
@@ -301,7 +318,7 @@ def main():
                     help="XCD run length of the single-lattice stream/collide block order (-1 automatic)")
     ap.add_argument("--workload", default="all",
                     choices=["all", "101", "201-f32", "201-f16", "201-bf16", "copy", "batch-f32", "batch-f16",
-                             "serve", "slab", "array"],
+                             "serve", "slab", "array", "cilia"],
                     help="profile one workload alone (no CPU baseline); 'copy' = the 1.23 GB calibration copy")
     args = ap.parse_args()
     if args.workload != "all":
@@ -329,12 +346,15 @@ def main():
                          f"{' then drained' if args.serve_drain else ''}, f16, async .dat output")
         print(json.dumps(r), flush=True)
         return
-    if args.workload == "array":
-        gels, c, f = gel_array_3x3(G, args.storage)
+    if args.workload in ("array", "cilia"):
+        gels, c, f = (gel_array_3x3 if args.workload == "array" else cilia_4x4)(G, args.storage)
         dt, ms, n = run_gpu(G, gels, c, f, args.steps, args.warmup)
-        print(json.dumps({"workload": "sim_array 3x3 gel array: 281x281x21 nodes, nine 20x20x4 gels",
-                          "markers": c.M, "value": round(f.N * f.nsub * args.steps / dt / 1e6, 1), "unit": "MLUPS",
+        name = ("sim_array 3x3 gel array: 281x281x21 nodes, nine 20x20x4 gels" if args.workload == "array" else
+                "sim_cilia 4x4 anchored gels: 201x201x61 nodes, sixteen 4x4x24 gels")
+        print(json.dumps({"workload": name, "markers": c.M,
+                          "value": round(f.N * f.nsub * args.steps / dt / 1e6, 1), "unit": "MLUPS",
                           "steps": args.steps, "ms_per_step": round(dt / args.steps * 1e3, 4),
+                          "s_for_200000_steps": round(dt / args.steps * 200000, 1),
                           "roofline": roofline(f.N, 2 * 19 * G.STORAGE_BYTES[args.storage], ms)}), flush=True)
         for x in gels + [c, f]:
             x.close()
