@@ -295,7 +295,7 @@ void Batch::enqueueStep(bool timed, bool lists) {
     const int sel = (pf_ + k) & 1;
     if (timed && k == nsub_ / 2) {
       hipEvent_t e[2];
-      for (auto& x : e) GLSM_HIP(hipEventCreate(&x));
+      for (auto& x : e) x = glsm::timing_event();
       glsm::launch_lbm_step_batch(storage_, t, sel, k == nsub_ - 1, shape, s, e[0], e[1]);
       timing_events_.push_back(e[0]);
       timing_events_.push_back(e[1]);
@@ -397,9 +397,7 @@ Batch::Launch Batch::autotune(int reps, double* best_ms) {
       for (int wgb : {2, 4}) cand.push_back(Launch{nt, run, 0, wgb, 1});
       for (int chunk : {2, 4, 8}) cand.push_back(Launch{nt, run, 0, 1, chunk});
     }
-  hipEvent_t e0, e1;
-  GLSM_HIP(hipEventCreate(&e0));
-  GLSM_HIP(hipEventCreate(&e1));
+  hipEvent_t e0 = glsm::timing_event(), e1 = glsm::timing_event();
   // one trial: a warm-up launch, then `reps` timed launches
   auto trial = [&](const Launch& c) {
     const glsm::LbmLaunch sh = to_internal(c);

@@ -665,9 +665,7 @@ int glsm_copy_bandwidth(size_t bytes, int reps, double* gbps) {
     hipStream_t s = glsm::stream();
     char* a = glsm::dalloc<char>(bytes);
     char* b = glsm::dalloc<char>(bytes);
-    hipEvent_t e0, e1;
-    GLSM_HIP(hipEventCreate(&e0));
-    GLSM_HIP(hipEventCreate(&e1));
+    hipEvent_t e0 = glsm::timing_event(), e1 = glsm::timing_event();
     glsm::launch_copy16(a, b, bytes, s);
     GLSM_HIP(hipEventRecord(e0, s));
     for (int r = 0; r < reps; ++r) glsm::launch_copy16(r % 2 ? b : a, r % 2 ? a : b, bytes, s);

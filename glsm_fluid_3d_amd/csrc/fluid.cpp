@@ -259,7 +259,7 @@ void Fluid::stepVelocity(int iter) {
     glsm::launch_ibm_interp(m, sl, d_u, beta_eff, fluid_stream);
     hipEvent_t ev[2] = {nullptr, nullptr};
     if (timed && kk == Nsub / 2) {
-      for (auto& e : ev) GLSM_HIP(hipEventCreate(&e));
+      for (auto& e : ev) e = glsm::timing_event();
       timing_events_.push_back(ev[0]);
       timing_events_.push_back(ev[1]);
     }

@@ -76,6 +76,15 @@ T* dalloc(size_t n) {  // zero-filled device allocation
 inline void dfree(void* p) {
   if (p) (void)hipFree(p);
 }
+// An event used only to time device work, read after the stream has been
+// synchronised: recorded without a system-scope release, which would write
+// back and invalidate the L2 at the timed launch's end and count that in its
+// duration (and slow the launches after it).
+inline hipEvent_t timing_event() {
+  hipEvent_t e = nullptr;
+  GLSM_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
+  return e;
+}
 
 // ---- gel --------------------------------------------------------------------
 // Everything a gel kernel reads: dimensions, constants, the 27-entry offset

@@ -59,7 +59,7 @@ Slab::Slab(const Fluid& like, int z0, int nz) : dev_(new glsm::SlabDevice()) {
   sl.nblk = nblk;
   sl.blkflag = flags_;
   s_ = glsm::stream();
-  for (auto& e : ev_) GLSM_HIP(hipEventCreate(&e));
+  for (auto& e : ev_) e = glsm::timing_event();
 }
 
 Slab::~Slab() {
