@@ -160,7 +160,7 @@ std::vector<char> Batch::memberParams(const Member& m) const {
   };
   const Fluid& f = *m.fluid;
   const glsm::LbmParams p = glsm::fluid_lbm_params(f);
-  const glsm::ScalarParams sp{f.h_fp->dt, f.h_fp->D, f.h_fp->dx, f.h_fp->dy, f.h_fp->dz};
+  const glsm::ScalarParams sp = glsm::scalar_params(f.h_fp->dt, f.h_fp->D, f.h_fp->dx, f.h_fp->dy, f.h_fp->dz);
   const float vals[6] = {m.coupler->h_cp->gel_to_lbm_vel, f.h_fp->beta, f.ramp_steps, m.coupler->h_cp->delta,
                          m.coupler->h_cp->gamma, 0.f};
   put(&p, sizeof p);
@@ -197,7 +197,7 @@ void Batch::rebuild() {
     glsm::BatchInst b{};
     b.g = glsm::fluid_geom(f);
     b.p = glsm::fluid_lbm_params(f);
-    b.sp = glsm::ScalarParams{f.h_fp->dt, f.h_fp->D, f.h_fp->dx, f.h_fp->dy, f.h_fp->dz};
+    b.sp = glsm::scalar_params(f.h_fp->dt, f.h_fp->D, f.h_fp->dx, f.h_fp->dy, f.h_fp->dz);
     b.f[pf_] = f.d_f;
     b.f[pf_ ^ 1] = f.d_fnext;
     b.u = f.d_u;

@@ -333,7 +333,7 @@ void Fluid::stepConcentration(int /*iter*/) {
   const glsm::LatticeGeom g = geom(*this);
   const glsm::MarkerArrays m = glsm::markers_of(*coupler);
   glsm::launch_ibm_sample(m, spreadLists(), d_c1, fluid_stream);
-  glsm::ScalarParams sp{h_fp->dt, h_fp->D, h_fp->dx, h_fp->dy, h_fp->dz};
+  const glsm::ScalarParams sp = glsm::scalar_params(h_fp->dt, h_fp->D, h_fp->dx, h_fp->dy, h_fp->dz);
   glsm::launch_scalar(g, sp, d_u, d_c1, d_c2, spreadLists(), m.Sl, fluid_stream);
   std::swap(d_c1, d_c2);
 }

@@ -183,6 +183,19 @@ struct LatticeGeom {
   double rNx, rNxNy;  // 1/Nx, 1/(Nx*Ny) for exact index decoding
 };
 
+// i = x + Nx (y + Ny z).  z = floor((i + 1/2) / (Nx Ny)), then y from the
+// remainder the same way: the quotient's fractional part lies at least
+// 1/(2n) from an integer, far more than the FP64 product's rounding error
+// (relative 2^-52 of a quotient below 2^51 / n), so truncating gives the
+// exact quotient without a correction step.
+__device__ __forceinline__ void decode_cell(const LatticeGeom& g, int i, int& x, int& y, int& z) {
+  const int sxy = g.Nx * g.Ny;
+  z = (int)fma((double)i, g.rNxNy, 0.5 * g.rNxNy);
+  const int r = i - z * sxy;
+  y = (int)fma((double)r, g.rNx, 0.5 * g.rNx);
+  x = r - (int)__umul24((unsigned)y, (unsigned)g.Nx);  // y * Nx < 2^32, both factors below 2^24
+}
+
 struct SpreadLists {
   int nblk;
   int* blkflag;
@@ -303,7 +316,11 @@ void launch_lbm_density(int storage, const LatticeGeom& g, const LbmParams& p, c
 
 struct ScalarParams {
   float dt, D, dx, dy, dz;
+  float ihx, ihy, ihz;  // 1/dx, 1/dy, 1/dz, correctly rounded (as the device's division would give)
 };
+inline ScalarParams scalar_params(float dt, float D, float dx, float dy, float dz) {
+  return ScalarParams{dt, D, dx, dy, dz, 1.0f / dx, 1.0f / dy, 1.0f / dz};
+}
 void launch_scalar(const LatticeGeom& g, const ScalarParams& p, const float3* u, const float* c1,
                    float* c2, const SpreadLists& sl, const float* Sl, hipStream_t s);
 

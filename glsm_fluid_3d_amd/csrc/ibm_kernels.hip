@@ -534,17 +534,11 @@ __device__ __forceinline__ void sort_at(const LatticeGeom& g, const SpreadLists&
 }
 
 // ---------------------------------------------------------------- scalar
-__device__ inline void decode_cell(const LatticeGeom& g, int i, int& x, int& y, int& z) {
-  // exact i / (Nx*Ny) and i / Nx via a double reciprocal and one correction
-  int q = (int)((double)i * g.rNxNy);
-  int r = i - q * g.Nx * g.Ny;
-  if (r < 0) { --q; r += g.Nx * g.Ny; } else if (r >= g.Nx * g.Ny) { ++q; r -= g.Nx * g.Ny; }
-  z = q;
-  int q2 = (int)((double)r * g.rNx);
-  int r2 = r - q2 * g.Nx;
-  if (r2 < 0) { --q2; r2 += g.Nx; } else if (r2 >= g.Nx) { ++q2; r2 -= g.Nx; }
-  y = q2;
-  x = r2;
+// Element `idx` of a lattice field (u as 3 floats per cell, or c): a 32-bit
+// byte offset from a scalar base (every field of a lattice whose
+// distributions fit the library's 2^32-element limit is below 4 GiB).
+__device__ __forceinline__ float field_at(const GLSM_GLOBAL float* f, unsigned idx) {
+  return *reinterpret_cast<const GLSM_GLOBAL float*>(reinterpret_cast<const GLSM_GLOBAL char*>(f) + idx * 4u);
 }
 
 // first-order upwind advection + 7-point diffusion + spread IBM source,
@@ -594,19 +588,27 @@ __device__ __forceinline__ void scalar_at(const LatticeGeom& g, const ScalarPara
         if (k + j < e1) src += sv[j] * w[j];
     }
   }
-  const float ihx = 1.0f / p.dx, ihy = 1.0f / p.dy, ihz = 1.0f / p.dz;
+  const float ihx = p.ihx, ihy = p.ihy, ihz = p.ihz;
   const float ihx2 = ihx * ihx, ihy2 = ihy * ihy, ihz2 = ihz * ihz;
-  const float c = gc[i];
-  const float cxp = gc[ixp], cxm = gc[ixm], cyp = gc[iyp], cym = gc[iym], czp = gc[izp], czm = gc[izm];
-  const float u0x = gu[3 * i], u0y = gu[3 * i + 1], u0z = gu[3 * i + 2];
-  const float uL = 0.5f * (gu[3 * ixm] + u0x), uR = 0.5f * (u0x + gu[3 * ixp]);
-  const float vL = 0.5f * (gu[3 * iym + 1] + u0y), vR = 0.5f * (u0y + gu[3 * iyp + 1]);
-  const float wL = 0.5f * (gu[3 * izm + 2] + u0z), wR = 0.5f * (u0z + gu[3 * izp + 2]);
+  const float c = field_at(gc, i);
+  const float cxp = field_at(gc, ixp), cxm = field_at(gc, ixm), cyp = field_at(gc, iyp), cym = field_at(gc, iym),
+              czp = field_at(gc, izp), czm = field_at(gc, izm);
+  // the cell's own u as one 12-byte access (three 4-byte loads would touch
+  // its lines three times)
+  struct U3 {
+    float x, y, z;
+  };
+  const U3 u0 = *reinterpret_cast<const GLSM_GLOBAL U3*>(reinterpret_cast<const GLSM_GLOBAL char*>(gu) + 12u * (unsigned)i);
+  const float u0x = u0.x, u0y = u0.y, u0z = u0.z;
+  const float uL = 0.5f * (field_at(gu, 3u * ixm) + u0x), uR = 0.5f * (u0x + field_at(gu, 3u * ixp));
+  const float vL = 0.5f * (field_at(gu, 3u * iym + 1) + u0y), vR = 0.5f * (u0y + field_at(gu, 3u * iyp + 1));
+  const float wL = 0.5f * (field_at(gu, 3u * izm + 2) + u0z), wR = 0.5f * (u0z + field_at(gu, 3u * izp + 2));
   const float FLx = uL * ((uL >= 0.0f) ? cxm : c), FLy = vL * ((vL >= 0.0f) ? cym : c), FLz = wL * ((wL >= 0.0f) ? czm : c);
   const float FRx = uR * ((uR >= 0.0f) ? c : cxp), FRy = vR * ((vR >= 0.0f) ? c : cyp), FRz = wR * ((wR >= 0.0f) ? c : czp);
   const float conv = -((FRx - FLx) * ihx + (FRy - FLy) * ihy + (FRz - FLz) * ihz);
   const float diff = p.D * ((cxm - 2.0f * c + cxp) * ihx2 + (cym - 2.0f * c + cyp) * ihy2 + (czm - 2.0f * c + czp) * ihz2);
-  gmem(c2)[i] = c + p.dt * (conv + diff + src);
+  *reinterpret_cast<GLSM_GLOBAL float*>(reinterpret_cast<GLSM_GLOBAL char*>(gmem(c2)) + (unsigned)i * 4u) =
+      c + p.dt * (conv + diff + src);
 }
 
 // ---------------------------------------------------------------- kernels

@@ -164,19 +164,6 @@ __device__ __forceinline__ float to_raw(float val, const LbmParams& p) {
   return Store<S>::scaled ? val * p.scale : val;
 }
 
-// i = x + Nx (y + Ny z).  z = floor((i + 1/2) / (Nx Ny)), then y from the
-// remainder the same way: the quotient's fractional part lies at least
-// 1/(2n) from an integer, far more than the FP64 product's rounding error
-// (relative 2^-52 of a quotient below 2^51 / n), so truncating gives the
-// exact quotient without a correction step.
-__device__ inline void decode_cell(const LatticeGeom& g, int i, int& x, int& y, int& z) {
-  const int sxy = g.Nx * g.Ny;
-  z = (int)fma((double)i, g.rNxNy, 0.5 * g.rNxNy);
-  const int r = i - z * sxy;
-  y = (int)fma((double)r, g.rNx, 0.5 * g.rNx);
-  x = r - (int)__umul24((unsigned)y, (unsigned)g.Nx);  // y * Nx < 2^32, both factors below 2^24
-}
-
 // Pull-source addressing of one cell: for each axis, the flat offset to the
 // upstream neighbour of a +1 / -1 velocity and whether it exists.  Offsets
 // are in units of E bytes: E = 1 gives cell offsets, E = sizeof(T) byte
