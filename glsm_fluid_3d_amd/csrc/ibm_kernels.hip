@@ -749,20 +749,36 @@ __global__ void __launch_bounds__(kThreads) kb_ibm_interp_t(BatchTables t) {
   const SpreadLists sl = I.sl;
   interp_component(m, sl, I.u, beta_eff_of(I, t.iters[t.mk_inst[gm]]), l, g3 % 3);
 }
-__global__ void __launch_bounds__(kThreads) kb_ibm_interp_s(BatchTables t) {
-  __shared__ InterpStage st;
-  const int base = blockIdx.x * kStageMarkers, mi = threadIdx.x % kStageMarkers, part = threadIdx.x / kStageMarkers;
-  const int gm = base + mi;
-  const bool live = gm < t.M;
-  int l = 0;
-  const BatchInst& I = marker_member(t, live ? gm : 0, l);
+// one work-group's staged interpolation of markers [base, base + 32) with
+// member I holding marker gm (member-local index l)
+__device__ __forceinline__ void interp_s_member(const BatchInst& I, int iter, int l, bool live, int part, int mi,
+                                                InterpStage& st) {
   const bool fin = part < 3 && live;
-  const float beta_eff = fin ? beta_eff_of(I, t.iters[t.mk_inst[gm]]) : 0.f;
+  const float beta_eff = fin ? beta_eff_of(I, iter) : 0.f;
   const float ws = fin ? gmem(I.sl.wsum)[l] : 0.f;
   const float vl = fin ? gmem(reinterpret_cast<const float*>(I.m.Vl))[3 * (size_t)l + part] : 0.f;
   if (live) interp_stage_load(I.sl, I.m.M, I.u, l, part, mi, st);
   __syncthreads();
   if (fin) interp_stage_finish(I.m, beta_eff, l, part, ws, vl, interp_stage_sum(st, part, mi));
+}
+__global__ void __launch_bounds__(kThreads) kb_ibm_interp_s(BatchTables t) {
+  __shared__ InterpStage st;
+  const int base = blockIdx.x * kStageMarkers, mi = threadIdx.x % kStageMarkers, part = threadIdx.x / kStageMarkers;
+  const int gm = base + mi;
+  const bool live = gm < t.M;
+  // Most work-groups' markers all belong to one member (members are
+  // contiguous in the marker numbering): its descriptor and iteration are
+  // then read once, as scalar loads, instead of per lane after a per-lane
+  // member lookup.
+  const int first = t.mk_inst[base], last = t.mk_inst[min(base + kStageMarkers, t.M) - 1];
+  if (first == last) {
+    const BatchInst& I = t.inst[first];
+    interp_s_member(I, t.iters[first], (live ? gm : base) - I.mk0, live, part, mi, st);
+  } else {
+    int l = 0;
+    const BatchInst& I = marker_member(t, live ? gm : 0, l);
+    interp_s_member(I, live ? t.iters[t.mk_inst[gm]] : 0, l, live, part, mi, st);
+  }
 }
 __global__ void __launch_bounds__(kThreads) kb_ibm_sample_t(BatchTables t, int cpar) {
   const int gm = blockIdx.x * blockDim.x + threadIdx.x;
