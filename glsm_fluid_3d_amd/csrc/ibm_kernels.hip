@@ -627,11 +627,12 @@ __global__ void __launch_bounds__(kThreads) k_gel_marker_box(MarkerArrays m, con
 }
 __global__ void __launch_bounds__(kThreads) k_repulsion_culled(MarkerArrays m, int ngel, const int* gel_off,
                                                                const float* box, float delta, float gamma) {
-  const int a = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  // one marker per wavefront: its own fields as scalar loads
+  const int a = __builtin_amdgcn_readfirstlane(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
   if (a < m.M) repulsion_wave_culled(m, ngel, gel_off, box, delta, gamma, a, threadIdx.x & 63);
 }
 __global__ void __launch_bounds__(kThreads) k_repulsion(MarkerArrays m, float delta, float gamma) {
-  const int a = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  const int a = __builtin_amdgcn_readfirstlane(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
   if (a < m.M) repulsion_wave(m, delta, gamma, a, threadIdx.x & 63);  // whole wavefront
 }
 __global__ void __launch_bounds__(kThreads) k_ibm_interp(MarkerArrays m, SpreadLists sl, const float3* __restrict__ u,
@@ -721,7 +722,8 @@ __global__ void __launch_bounds__(kThreads) kb_scatter(BatchTables t) {
   scatter_at(I.m, l);
 }
 __global__ void __launch_bounds__(kThreads) kb_repulsion(BatchTables t) {
-  const int gm = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  // one marker per wavefront: its member lookup and own fields as scalar loads
+  const int gm = __builtin_amdgcn_readfirstlane(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
   if (gm >= t.M) return;
   int l;
   const BatchInst& I = marker_member(t, gm, l);
@@ -781,8 +783,16 @@ __global__ void __launch_bounds__(kThreads) kb_ibm_interp_s(BatchTables t) {
   }
 }
 __global__ void __launch_bounds__(kThreads) kb_ibm_sample_t(BatchTables t, int cpar) {
-  const int gm = blockIdx.x * blockDim.x + threadIdx.x;
+  const int base = blockIdx.x * blockDim.x, gm = base + threadIdx.x;
+  // one member for the whole work-group (the usual case): its descriptor as
+  // scalar loads, as in the staged interpolation
+  const int first = t.mk_inst[base], last = t.mk_inst[min(base + (int)blockDim.x, t.M) - 1];
   if (gm >= t.M) return;
+  if (first == last) {
+    const BatchInst& I = t.inst[first];
+    sample_thread(I.m, I.sl, I.c[cpar], gm - I.mk0);
+    return;
+  }
   int l;
   const BatchInst& I = marker_member(t, gm, l);
   const MarkerArrays m = I.m;
