@@ -428,7 +428,10 @@ __device__ __forceinline__ void collide_store(const float (&v)[19], float Fx, fl
 // Every access goes through GLSM_GLOBAL pointers: the batched kernels take
 // their pointers from a device-side table, which would otherwise compile to
 // flat memory instructions.
-template <int S, bool NT, bool B32>
+// PRIO: the wavefront runs at raised priority while it forms addresses and
+// issues its 19 loads, and at normal priority for the collision, so that
+// wavefronts about to load get issue slots ahead of those computing
+template <int S, bool NT, bool B32, bool PRIO = false>
 __device__ __forceinline__ void lbm_block(const LatticeGeom& g, const LbmParams& p,
                                           const typename Store<S>::T* __restrict__ fin,
                                           typename Store<S>::T* __restrict__ fout, float3* __restrict__ u,
@@ -438,6 +441,7 @@ __device__ __forceinline__ void lbm_block(const LatticeGeom& g, const LbmParams&
   const unsigned i = b * kThreads + lane;
   const int flagged = gmem(sl.blkflag)[b];
   if (i >= (unsigned)g.N) return;
+  if constexpr (PRIO) __builtin_amdgcn_s_setprio(3);
   int x, y, z;
   decode_cell(g, (int)i, x, y, z);
   using T = typename Store<S>::T;
@@ -463,6 +467,7 @@ __device__ __forceinline__ void lbm_block(const LatticeGeom& g, const LbmParams&
       v[q] = Store<S>::ld(gin + a.src<q>(i));
     });
   }
+  if constexpr (PRIO) __builtin_amdgcn_s_setprio(0);
   bool store_u = write_u_all;
   GLSM_GLOBAL Vec3* gu = gmem(reinterpret_cast<Vec3*>(u));
   if (flagged || const_force) {
@@ -518,7 +523,8 @@ __global__ void __launch_bounds__(kThreads, 8) k_lbm_step(LatticeGeom g, LbmPara
                                                        float3* __restrict__ u, int write_u_all, SpreadLists sl,
                                                        const float3* __restrict__ Fl, int const_force) {
   const unsigned b = xcd_block<RUN>(blockIdx.x, gridDim.x);
-  lbm_block<S, NT, B32>(g, p, fin, fout, u, write_u_all, sl, Fl, const_force, b, threadIdx.x);
+  lbm_block<S, NT, B32, sizeof(typename Store<S>::T) == 2>(g, p, fin, fout, u, write_u_all, sl, Fl, const_force, b,
+                                                            threadIdx.x);
 }
 
 // All members of a batch: the blocks of every lattice form one flattened
