@@ -198,6 +198,9 @@ __device__ __forceinline__ void decode_cell(const LatticeGeom& g, int i, int& x,
 
 struct SpreadLists {
   int nblk;
+  // per 256-cell block: byte w (of 4) is nonzero when some cell of the
+  // block's w-th run of 64 cells (one wavefront) has a contribution, so the
+  // int is nonzero when any cell of the block has one
   int* blkflag;
   int* cnt;
   int* fillpos;
@@ -211,6 +214,13 @@ struct SpreadLists {
   float* rtab;   // interpolation weight phi_x phi_y phi_z
   float* wsum;   // per marker: sum of rtab in stencil order
 };
+
+// the flag byte, in a block's flag word, of the wavefront holding `lane`
+// (0..255), as a wavefront-uniform value
+__device__ __forceinline__ int wave_flag(int flag_word, unsigned lane) {
+  const unsigned shift = __builtin_amdgcn_readfirstlane(8u * (lane >> 6));
+  return (int)(((unsigned)flag_word >> shift) & 0xffu);
+}
 
 // rebuild the per-cell contribution lists for the current marker positions
 void launch_spread_rebuild(const LatticeGeom& g, const MarkerArrays& m, const SpreadLists& sl,

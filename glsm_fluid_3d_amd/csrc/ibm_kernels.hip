@@ -443,7 +443,8 @@ __device__ __forceinline__ void spread_count_wave(const LatticeGeom& g, const Ma
   sl.rtab[k] = st.w;
   sl.wtab[k] = st.w * scale;
   atomicAdd(&sl.cnt[st.cell], 1);
-  sl.blkflag[st.cell / kFlagCells] = 1;
+  // the flag byte of the cell's wavefront (every writer stores the same 1)
+  reinterpret_cast<unsigned char*>(sl.blkflag + st.cell / kFlagCells)[(st.cell % kFlagCells) / 64] = 1;
 }
 
 __device__ __forceinline__ void clear_flag_at(const SpreadLists& sl, int b) { sl.blkflag[b] = 0; }
@@ -549,7 +550,7 @@ __device__ __forceinline__ void scalar_at(const LatticeGeom& g, const ScalarPara
                                           const float* __restrict__ c1, float* __restrict__ c2, const SpreadLists& sl,
                                           const float* __restrict__ Sl, int b) {
   const int i = b * kFlagCells + threadIdx.x;
-  const int flagged = gmem(sl.blkflag)[b];
+  const int flagged = wave_flag(gmem(sl.blkflag)[b], threadIdx.x);
   if (i >= g.N) return;
   int x, y, z;
   decode_cell(g, i, x, y, z);

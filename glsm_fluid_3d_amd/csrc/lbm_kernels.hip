@@ -439,7 +439,7 @@ __device__ __forceinline__ void lbm_block(const LatticeGeom& g, const LbmParams&
                                           const float3* __restrict__ Fl, int const_force, unsigned b,
                                           unsigned lane) {
   const unsigned i = b * kThreads + lane;
-  const int flagged = gmem(sl.blkflag)[b];
+  const int flag_word = gmem(sl.blkflag)[b];  // loaded first, tested after the population loads
   if (i >= (unsigned)g.N) return;
   if constexpr (PRIO) __builtin_amdgcn_s_setprio(3);
   int x, y, z;
@@ -468,6 +468,7 @@ __device__ __forceinline__ void lbm_block(const LatticeGeom& g, const LbmParams&
     });
   }
   if constexpr (PRIO) __builtin_amdgcn_s_setprio(0);
+  const int flagged = wave_flag(flag_word, lane);
   bool store_u = write_u_all;
   GLSM_GLOBAL Vec3* gu = gmem(reinterpret_cast<Vec3*>(u));
   if (flagged || const_force) {
