@@ -12,8 +12,9 @@
 //      a work-group's cells store one contiguous region;
 //   2. forms rho, u = (sum f c + F/2)/rho with F = F_const + F_ibm, where
 //      F_ibm is gathered from the marker forces through the cell's sorted
-//      (marker, weight) list -- only in 256-cell blocks that some marker
-//      stencil touches (one uniform flag load per block);
+//      (marker, weight) list -- only in the 64-cell wavefronts that some
+//      marker stencil touches (one flag byte per wavefront, loaded with the
+//      block's flag word);
 //   3. relaxes to the second-order equilibrium with the Guo forcing term
 //      (fluid_kernels.cu:98-142) and stores the 19 post-collision values.
 // Algorithmic traffic per cell update: 19 loads + 19 stores of the storage
@@ -516,7 +517,9 @@ __device__ __forceinline__ void lbm_block(const LatticeGeom& g, const LbmParams&
 }
 
 // 8 waves per SIMD (<= 64 VGPRs): keeps enough loads in flight per CU.
-// RUN: the XCD run length of the block order (xcd_block).
+// RUN: the XCD run length of the block order (xcd_block).  16-bit storages
+// raise the wave priority around the loads (measured: +0.7% at 201^3; the
+// FP32 and batched kernels were slower with it).
 template <int S, bool NT, bool B32, int RUN>
 __global__ void __launch_bounds__(kThreads, 8) k_lbm_step(LatticeGeom g, LbmParams p,
                                                        const typename Store<S>::T* __restrict__ fin,
